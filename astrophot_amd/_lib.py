@@ -1,0 +1,110 @@
+"""ctypes binding of libastrophot_hip.so (the C ABI in include/astrophot_hip.h).
+
+torch is imported before the library is opened so that the library's
+``libamdhip64.so.7`` dependency resolves to the HIP runtime torch already
+loaded: one runtime per process, so torch's streams and allocations are valid
+arguments.  There is no fallback: if the library is missing or no GPU is
+visible, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the dlopen below)
+
+from .errors import APXError
+
+LIB_PATH = Path(__file__).resolve().parent / "libastrophot_hip.so"
+
+APX_SERSIC, APX_EXPONENTIAL, APX_GAUSSIAN, APX_MOFFAT, APX_FLAT_SKY = range(5)
+APX_P_CX, APX_P_CY, APX_P_Q, APX_P_PA, APX_P_A, APX_P_B, APX_P_C = range(7)
+APX_NSLOT = 7
+APX_MAX_CHAN = 8
+APX_SAMPLE_MIDPOINT, APX_SAMPLE_QUAD = 0, 1
+APX_MAX_QUAD_LEVEL = 7
+APX_MAX_GRIDDING = 8
+APX_MAX_DEPTH = 3
+APX_MAX_KERNEL = 131
+
+_i32, _i64, _f64, _p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class Component(ctypes.Structure):
+    """apx_component."""
+
+    _fields_ = [
+        ("profile", _i32), ("sampling", _i32), ("sample_quad", _i32), ("integrate", _i32),
+        ("quad_level", _i32), ("gridding", _i32), ("max_depth", _i32), ("mean_reference", _i32),
+        ("nx", _i32), ("ny", _i32), ("ci", _i32), ("cj", _i32), ("nchan", _i32),
+        ("chan_slot", _i32 * APX_MAX_CHAN),
+        ("P", _f64 * 4), ("frac", _f64 * 2), ("pixel_area", _f64), ("softening", _f64),
+        ("tolerance", _f64), ("reference", _f64), ("q", _f64), ("pa", _f64), ("prm", _f64 * 3),
+        ("dvdr", _f64 * APX_NSLOT),
+        ("gl_node", _f64 * 8), ("gl_weight", _f64 * 8), ("sq_node", _f64 * 8), ("sq_weight", _f64 * 8),
+    ]
+
+
+class ConvJob(ctypes.Structure):
+    """apx_conv_job."""
+
+    _fields_ = [
+        ("nx", _i32), ("ny", _i32), ("nchan", _i32), ("kh", _i32), ("kw", _i32),
+        ("x0", _i32), ("y0", _i32), ("ow", _i32), ("oh", _i32),
+        ("img", _p), ("img_row", _i64), ("img_w", _i32), ("img_h", _i32), ("img_x0", _i32), ("img_y0", _i32),
+        ("jac", _p), ("jac_row", _i64), ("jac_w", _i32), ("jac_h", _i32), ("jac_np", _i32),
+        ("jac_x0", _i32), ("jac_y0", _i32),
+        ("chan_col", _i32 * APX_MAX_CHAN), ("dk_col", _i32 * 2),
+    ]
+
+
+# name -> (restype, argtypes); every symbol declared in include/astrophot_hip.h
+SIGNATURES = {
+    "apx_last_error": (ctypes.c_char_p, []),
+    "apx_render_workspace_bytes": (_i64, [_i64]),
+    "apx_render_component": (ctypes.c_int, [ctypes.POINTER(Component), _p, _p, _i64, _p]),
+    "apx_render_selected_count": (ctypes.c_int, [_p, _p, ctypes.POINTER(_i32)]),
+    "apx_psf_shift": (ctypes.c_int, [_p, _i32, _i32, _f64, _f64, _p, _f64, _f64, _p, _p]),
+    "apx_convolve": (ctypes.c_int, [ctypes.POINTER(ConvJob), _p, _p, _p]),
+    "apx_reduce_workspace_bytes": (_i64, [_i64, _i32]),
+    "apx_chi2": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p]),
+    "apx_normal_equations": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i32, _p, _p, _p, _i64, _p]),
+    "apx_rpp": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i32, _p, _f64, _p, _p, _i64, _p]),
+    "apx_lm_solve": (ctypes.c_int, [_p, _p, _i32, _f64, _p, _p, _p]),
+}
+
+_lib = None
+
+
+def load(require_gpu: bool = True):
+    """Open the library (once).  Raises APXError if it is missing, or if
+    ``require_gpu`` and torch sees no GPU."""
+    global _lib
+    if require_gpu and not torch.cuda.is_available():
+        raise APXError("astrophot_amd needs a HIP GPU: torch.cuda.is_available() is False")
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise APXError(f"{LIB_PATH.name} is not built; run `python -m astrophot_amd._build`")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load(require_gpu=False).apx_last_error().decode()
+        raise APXError(f"libastrophot_hip error {rc}: {msg}")
+
+
+def ptr(t) -> int:
+    """Device address of a tensor (None -> NULL)."""
+    return 0 if t is None else t.data_ptr()
+
+
+def stream() -> int:
+    return torch.cuda.current_stream().cuda_stream

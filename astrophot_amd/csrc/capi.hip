@@ -1,0 +1,89 @@
+// extern "C" entry points of libastrophot_hip (declared in
+// include/astrophot_hip.h) and the thread-local error record.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "capi_internal.h"
+
+namespace apx {
+
+namespace {
+thread_local char g_last_error[512] = "";
+}
+
+int set_error(int code, const char* msg) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s", msg);
+  return code;
+}
+
+int check_launch(const char* where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s", where, hipGetErrorString(e));
+    return APX_E_HIP;
+  }
+  return APX_OK;
+}
+
+}  // namespace apx
+
+extern "C" {
+
+const char* apx_last_error(void) { return apx::g_last_error; }
+
+int64_t apx_render_workspace_bytes(int64_t npix) { return apx::render_workspace_bytes(npix); }
+
+int apx_render_component(const apx_component* comp, float* planes, void* workspace,
+                         int64_t workspace_bytes, void* stream) {
+  return apx::render_component(comp, planes, workspace, workspace_bytes,
+                               static_cast<hipStream_t>(stream));
+}
+
+int apx_render_selected_count(const void* workspace, void* stream, int32_t* count) {
+  return apx::render_selected_count(workspace, static_cast<hipStream_t>(stream), count);
+}
+
+int apx_psf_shift(const float* psf, int32_t ph, int32_t pw, double shift_x, double shift_y,
+                  const double* pinv, double dvdr_x, double dvdr_y, float* kernels,
+                  void* stream) {
+  return apx::psf_shift(psf, ph, pw, shift_x, shift_y, pinv, dvdr_x, dvdr_y, kernels,
+                        static_cast<hipStream_t>(stream));
+}
+
+int apx_convolve(const apx_conv_job* job, const float* planes, const float* kernels,
+                 void* stream) {
+  return apx::convolve(job, planes, kernels, static_cast<hipStream_t>(stream));
+}
+
+int64_t apx_reduce_workspace_bytes(int64_t npix, int32_t np) {
+  return apx::reduce_workspace_bytes(npix, np);
+}
+
+int apx_chi2(const float* model, const float* data, const float* weight, const uint8_t* mask,
+             int64_t npix, double* out, void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::chi2(model, data, weight, mask, npix, out, workspace, workspace_bytes,
+                   static_cast<hipStream_t>(stream));
+}
+
+int apx_normal_equations(const float* J, const float* model, const float* data, const float* weight,
+                         const uint8_t* mask, int64_t npix, int32_t np, double* hess, double* grad,
+                         void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::normal_equations(J, model, data, weight, mask, npix, np, hess, grad, workspace,
+                               workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int apx_rpp(const float* J, const float* m0, const float* m1, const float* weight,
+            const uint8_t* mask, int64_t npix, int32_t np, const double* h, double d, double* rpp,
+            void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::rpp(J, m0, m1, weight, mask, npix, np, h, d, rpp, workspace, workspace_bytes,
+                  static_cast<hipStream_t>(stream));
+}
+
+int apx_lm_solve(const double* hess, const double* grad, int32_t np, double L, double* h,
+                 int32_t* status, void* stream) {
+  return apx::lm_solve(hess, grad, np, L, h, status, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

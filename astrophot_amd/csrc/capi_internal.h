@@ -1,0 +1,40 @@
+// Internal declarations shared by the .hip translation units of libastrophot_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "../../include/astrophot_hip.h"
+
+namespace apx {
+
+// Records `msg` as the calling thread's last error and returns `code`.
+int set_error(int code, const char* msg);
+// APX_OK, or APX_E_HIP with the runtime's message if the last launch failed.
+int check_launch(const char* where);
+
+int64_t render_workspace_bytes(int64_t npix);
+int render_component(const apx_component* cp, float* planes, void* workspace, int64_t ws_bytes,
+                     hipStream_t stream);
+int render_selected_count(const void* workspace, hipStream_t stream, int32_t* count);
+
+int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
+              double dvdr_x, double dvdr_y, float* kernels, hipStream_t stream);
+int convolve(const apx_conv_job* job, const float* planes, const float* kernels,
+             hipStream_t stream);
+
+int64_t reduce_workspace_bytes(int64_t npix, int32_t np);
+int chi2(const float* model, const float* data, const float* weight, const uint8_t* mask,
+         int64_t npix, double* out, void* ws, int64_t ws_bytes, hipStream_t stream);
+int normal_equations(const float* J, const float* model, const float* data, const float* weight,
+                     const uint8_t* mask, int64_t npix, int32_t np, double* hess, double* grad,
+                     void* ws, int64_t ws_bytes, hipStream_t stream);
+int rpp(const float* J, const float* m0, const float* m1, const float* weight, const uint8_t* mask,
+        int64_t npix, int32_t np, const double* h, double d, double* out, void* ws,
+        int64_t ws_bytes, hipStream_t stream);
+int lm_solve(const double* hess, const double* grad, int32_t np, double L, double* h,
+             int32_t* status, hipStream_t stream);
+
+}  // namespace apx

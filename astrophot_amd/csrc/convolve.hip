@@ -1,0 +1,233 @@
+// PSF sub-pixel shift and direct convolution on gfx950.
+//
+// psf_shift restates models/_model_methods.py:187-207 and :243 (bilinear
+// _shift_psf with utils/interpolate.py:282-329's clamped corners, which
+// extrapolate at the edge, then division by the sum) and adds the derivative
+// of the normalised kernel with respect to the model centre, the only path by
+// which the reference's forward-mode Jacobian sees the centre when
+// psf_mode="full" (the sub-pixel shift s = c - round(c) moves the PSF, the
+// sampling grid itself is centred on round(c); model_object.py:320-325).
+//
+// convolve restates models/_model_methods.py:233-257 and operations.py:9-36:
+// the FFT path there is a circular convolution of the border-padded working
+// image, and on the pixels kept after the crop (model_object.py:348-349) it is
+// exactly the linear convolution computed here directly.  The working-image
+// tile plus halo is staged in LDS once per render plane; kernel taps are
+// wave-uniform and come through the scalar cache; each thread keeps RY
+// vertically adjacent outputs in registers so every LDS read feeds up to RY
+// (value) or 3*RY (value + two centre derivatives) FMAs.  The crop, the
+// reduce(1) and the window-aware `image +=` (image_object.py:450-457) are the
+// epilogue.
+#include <hip/hip_runtime.h>
+
+#include "capi_internal.h"
+
+namespace apx {
+
+namespace {
+
+constexpr int kTX = 64;   // output tile width = one wavefront
+constexpr int kRY = 4;    // outputs per thread (vertical)
+constexpr int kTY = 16;   // output tile height (4 waves x kRY)
+constexpr int kKR = 32;   // kernel rows staged per pass
+
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__device__ __forceinline__ void bilinear(const float* psf, int ph, int pw, int r, int c, double sx,
+                                         double sy, double& val, double& dsx, double& dsy) {
+  const int H2 = ph + 2, W2 = pw + 2;
+  const double x = (double)c - sx, y = (double)r - sy;
+  const int xf = (int)floor(x), yf = (int)floor(y);
+  const int x0 = min(max(xf, 0), W2 - 2), x1 = min(max(xf + 1, 1), W2 - 1);
+  const int y0 = min(max(yf, 0), H2 - 2), y1 = min(max(yf + 1, 1), H2 - 1);
+  auto at = [&](int yy, int xx) -> double {
+    return (yy >= 1 && yy <= ph && xx >= 1 && xx <= pw) ? (double)psf[(yy - 1) * pw + (xx - 1)]
+                                                        : 0.0;
+  };
+  const double fa = at(y0, x0), fb = at(y1, x0), fc = at(y0, x1), fd = at(y1, x1);
+  const double wx1 = (double)x1 - x, wx0 = x - (double)x0;
+  const double wy1 = (double)y1 - y, wy0 = y - (double)y0;
+  val = fa * wx1 * wy1 + fb * wx1 * wy0 + fc * wx0 * wy1 + fd * wx0 * wy0;
+  dsx = -(-fa * wy1 - fb * wy0 + fc * wy1 + fd * wy0);  // d/ds_x = -d/dx
+  dsy = -(-fa * wx1 + fb * wx1 - fc * wx0 + fd * wx0);  // d/ds_y = -d/dy
+}
+
+__global__ __launch_bounds__(256) void psf_shift_kernel(const float* psf, int ph, int pw, double sx,
+                                                        double sy, double p0, double p1, double p2,
+                                                        double p3, double dvx, double dvy,
+                                                        float* out) {
+  __shared__ double red[4];
+  const int H2 = ph + 2, W2 = pw + 2, n = H2 * W2;
+  double S = 0.0, Sx = 0.0, Sy = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    double v, dx, dy;
+    bilinear(psf, ph, pw, k / W2, k % W2, sx, sy, v, dx, dy);
+    S += v;
+    Sx += dx;
+    Sy += dy;
+  }
+  S = block_sum256(S, red);
+  Sx = block_sum256(Sx, red);
+  Sy = block_sum256(Sy, red);
+  const double iS = 1.0 / S;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    const int r = k / W2, c = k % W2;
+    double v, dx, dy;
+    bilinear(psf, ph, pw, r, c, sx, sy, v, dx, dy);
+    const double K = v * iS;
+    const double Kx = dx * iS - v * Sx * iS * iS;  // d(v/S)/ds_x
+    const double Ky = dy * iS - v * Sy * iS * iS;
+    const int f = (H2 - 1 - r) * W2 + (W2 - 1 - c);  // flipped for correlation
+    out[f] = (float)K;
+    out[n + f] = (float)((Kx * p0 + Ky * p2) * dvx);
+    out[2 * n + f] = (float)((Kx * p1 + Ky * p3) * dvy);
+  }
+}
+
+// One workgroup = kTY x kTX outputs for every render plane.
+template <bool DK>
+__global__ __launch_bounds__(256) void convolve_kernel(apx_conv_job job, const float* __restrict__ planes,
+                                                       const float* __restrict__ kernels) {
+  extern __shared__ float tile[];
+  const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  const int ox0 = blockIdx.x * kTX, oy0 = blockIdx.y * kTY;
+  const int kh = job.kh, kw = job.kw, ch = (kh - 1) / 2, cw = (kw - 1) / 2;
+  const int KR = min(kh, kKR);
+  const int LW = kTX + kw - 1;                 // staged row length
+  const int gx0 = job.x0 + ox0 - cw;           // working-grid column of tile[.][0]
+  const int64_t npl = (int64_t)job.nx * job.ny;
+  const int knum = kh * kw;
+  const int ox = ox0 + lx;
+  for (int c = 0; c < job.nchan; ++c) {
+    const bool centre = DK && c == 0;
+    float acc[kRY], accx[kRY], accy[kRY];
+#pragma unroll
+    for (int r = 0; r < kRY; ++r) acc[r] = accx[r] = accy[r] = 0.0f;
+    const float* plane = planes + c * npl;
+    for (int kr0 = 0; kr0 < kh; kr0 += KR) {
+      const int kr = min(KR, kh - kr0);
+      const int rows = kTY + kr - 1;
+      const int gy0 = job.y0 + oy0 - ch + kr0;   // working-grid row of tile[0][.]
+      __syncthreads();
+      for (int e = threadIdx.x; e < rows * LW; e += 256) {
+        const int ty = e / LW, tx = e % LW;
+        const int gy = gy0 + ty, gx = gx0 + tx;
+        tile[e] = (gy >= 0 && gy < job.ny && gx >= 0 && gx < job.nx)
+                      ? plane[(int64_t)gy * job.nx + gx]
+                      : 0.0f;
+      }
+      __syncthreads();
+      const float* K0 = kernels + kr0 * kw;
+      for (int t = 0; t < kRY + kr - 1; ++t) {
+        const float* row = tile + (ly * kRY + t) * LW + lx;
+        for (int b = 0; b < kw; ++b) {
+          const float v = row[b];
+#pragma unroll
+          for (int r = 0; r < kRY; ++r) {
+            const int a = t - r;
+            if (a >= 0 && a < kr) {
+              acc[r] = fmaf(K0[a * kw + b], v, acc[r]);
+              if (centre) {
+                accx[r] = fmaf(K0[knum + a * kw + b], v, accx[r]);
+                accy[r] = fmaf(K0[2 * knum + a * kw + b], v, accy[r]);
+              }
+            }
+          }
+        }
+      }
+    }
+    if (ox < job.ow) {
+#pragma unroll
+      for (int r = 0; r < kRY; ++r) {
+        const int oy = oy0 + ly * kRY + r;
+        if (oy >= job.oh) continue;
+        if (c == 0) {
+          if (job.img) {
+            float* d = job.img + (int64_t)(job.img_y0 + oy) * job.img_row + (job.img_x0 + ox);
+            *d += acc[r];
+          }
+          if (centre && job.jac) {
+            float* d = job.jac + (int64_t)(job.jac_y0 + oy) * job.jac_row +
+                       (int64_t)(job.jac_x0 + ox) * job.jac_np;
+            if (job.dk_col[0] >= 0) d[job.dk_col[0]] += accx[r];
+            if (job.dk_col[1] >= 0) d[job.dk_col[1]] += accy[r];
+          }
+        } else if (job.jac) {
+          float* d = job.jac + (int64_t)(job.jac_y0 + oy) * job.jac_row +
+                     (int64_t)(job.jac_x0 + ox) * job.jac_np;
+          d[job.chan_col[c]] += acc[r];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
+              double dvdr_x, double dvdr_y, float* kernels, hipStream_t stream) {
+  if (!psf || !kernels || !pinv) return set_error(APX_E_INVALID, "psf_shift: null pointer");
+  if (ph < 1 || pw < 1 || (ph % 2) == 0 || (pw % 2) == 0)
+    return set_error(APX_E_INVALID, "psf_shift: psf must have odd shape");
+  if (ph + 2 > APX_MAX_KERNEL || pw + 2 > APX_MAX_KERNEL)
+    return set_error(APX_E_LIMIT, "psf_shift: psf larger than APX_MAX_KERNEL - 2");
+  hipLaunchKernelGGL(psf_shift_kernel, dim3(1), dim3(256), 0, stream, psf, ph, pw, sx, sy, pinv[0],
+                     pinv[1], pinv[2], pinv[3], dvdr_x, dvdr_y, kernels);
+  return check_launch("psf_shift");
+}
+
+int convolve(const apx_conv_job* jp, const float* planes, const float* kernels,
+             hipStream_t stream) {
+  if (!jp || !planes || !kernels) return set_error(APX_E_INVALID, "convolve: null pointer");
+  const apx_conv_job job = *jp;
+  if (job.kh < 1 || job.kw < 1 || (job.kh % 2) == 0 || (job.kw % 2) == 0)
+    return set_error(APX_E_INVALID, "convolve: kernel must have odd shape");
+  if (job.kh > APX_MAX_KERNEL || job.kw > APX_MAX_KERNEL)
+    return set_error(APX_E_LIMIT, "convolve: kernel larger than APX_MAX_KERNEL");
+  if (job.nchan < 1 || job.nchan > APX_MAX_CHAN)
+    return set_error(APX_E_INVALID, "convolve: nchan out of range");
+  if (job.ow < 0 || job.oh < 0) return set_error(APX_E_INVALID, "convolve: negative box");
+  if (job.ow == 0 || job.oh == 0) return APX_OK;
+  // every output reads inside the working grid
+  const int ch = (job.kh - 1) / 2, cw = (job.kw - 1) / 2;
+  if (job.x0 - cw < 0 || job.y0 - ch < 0 || job.x0 + job.ow - 1 + cw >= job.nx ||
+      job.y0 + job.oh - 1 + ch >= job.ny)
+    return set_error(APX_E_INVALID, "convolve: output box reads outside the working grid");
+  if (job.img) {
+    if (job.img_row < job.img_w || job.img_x0 < 0 || job.img_y0 < 0 ||
+        job.img_x0 + job.ow > job.img_w || job.img_y0 + job.oh > job.img_h)
+      return set_error(APX_E_INVALID, "convolve: output box outside the image");
+  }
+  if (job.jac) {
+    if (job.jac_np < 1 || job.jac_row < (int64_t)job.jac_w * job.jac_np || job.jac_x0 < 0 ||
+        job.jac_y0 < 0 || job.jac_x0 + job.ow > job.jac_w || job.jac_y0 + job.oh > job.jac_h)
+      return set_error(APX_E_INVALID, "convolve: output box outside the Jacobian");
+  }
+  const bool dk = job.dk_col[0] >= 0 || job.dk_col[1] >= 0;
+  if (job.nchan > 1 || dk) {
+    if (!job.jac || job.jac_np < 1) return set_error(APX_E_INVALID, "convolve: no Jacobian");
+    for (int c = 1; c < job.nchan; ++c)
+      if (job.chan_col[c] < 0 || job.chan_col[c] >= job.jac_np)
+        return set_error(APX_E_INVALID, "convolve: channel column out of range");
+    for (int k = 0; k < 2; ++k)
+      if (job.dk_col[k] >= job.jac_np)
+        return set_error(APX_E_INVALID, "convolve: centre column out of range");
+  }
+  const int KR = std::min(job.kh, kKR);
+  const size_t lds = sizeof(float) * (size_t)(kTY + KR - 1) * (size_t)(kTX + job.kw - 1);
+  const dim3 grid((unsigned)((job.ow + kTX - 1) / kTX), (unsigned)((job.oh + kTY - 1) / kTY));
+  if (dk)
+    hipLaunchKernelGGL(convolve_kernel<true>, grid, dim3(256), lds, stream, job, planes, kernels);
+  else
+    hipLaunchKernelGGL(convolve_kernel<false>, grid, dim3(256), lds, stream, job, planes, kernels);
+  return check_launch("convolve");
+}
+
+}  // namespace apx

@@ -1,0 +1,188 @@
+// Device-side profile evaluation for the gfx950 render kernels.
+//
+// A galaxy component is evaluated at a plane offset (X, Y) from its centre:
+//   rotate by theta = -(PA - pi/2)       (models/_shared_methods.py:301-307,
+//                                         image/image_header.py:36)
+//   Y' /= q, R = sqrt(X'^2 + Y'^2 + softening^2)   (_model_methods.py:38-40)
+//   I(R) = profile(R) * pixel_area * 10**log-brightness
+//                                        (_shared_methods.py:312-408,
+//                                         utils/parametric_profiles.py:7-100)
+// together with the analytic partials dI/d(slot) that forward-mode AD produces
+// in the reference (_model_methods.py:325-340), slots as in APX_P_*.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/astrophot_hip.h"
+
+namespace apx {
+
+constexpr double kLn10 = 2.30258509299404568402;
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ __forceinline__ float f_exp(float x) { return __expf(x); }
+__device__ __forceinline__ double f_exp(double x) { return exp(x); }
+__device__ __forceinline__ float f_log(float x) { return __logf(x); }
+__device__ __forceinline__ double f_log(double x) { return log(x); }
+__device__ __forceinline__ float f_sqrt(float x) { return __fsqrt_rn(x); }
+__device__ __forceinline__ double f_sqrt(double x) { return sqrt(x); }
+
+// sersic_n_to_b and its derivative in n (utils/conversions/functions.py:7-22)
+__host__ __device__ inline double sersic_b(double n) {
+  return 2.0 * n - 1.0 / 3.0 + 4.0 / (405.0 * n) + 46.0 / (25515.0 * n * n) +
+         131.0 / (1148175.0 * n * n * n) - 2194697.0 / (30690717750.0 * n * n * n * n);
+}
+__host__ __device__ inline double sersic_db(double n) {
+  return 2.0 - 4.0 / (405.0 * n * n) - 2.0 * 46.0 / (25515.0 * n * n * n) -
+         3.0 * 131.0 / (1148175.0 * n * n * n * n) +
+         4.0 * 2194697.0 / (30690717750.0 * n * n * n * n * n);
+}
+
+// Per-component constants in the arithmetic type T.
+template <typename T>
+struct Prof {
+  int kind;
+  T P00, P01, P10, P11, fx, fy;  // X = P (di, dj) + frac
+  T c, s, q, invq, qm;            // rotation, axis ratio, q - 1/q
+  T soft2;
+  T amp;                          // level-1 pixel_area * 10**I
+  T a, inva, b, db, binva, inv_s; // profile-specific
+};
+
+template <typename T>
+__device__ inline Prof<T> make_prof(const apx_component& cp) {
+  Prof<T> p;
+  p.kind = cp.profile;
+  p.P00 = (T)cp.P[0];
+  p.P01 = (T)cp.P[1];
+  p.P10 = (T)cp.P[2];
+  p.P11 = (T)cp.P[3];
+  p.fx = (T)cp.frac[0];
+  p.fy = (T)cp.frac[1];
+  const double theta = -(cp.pa - kPi / 2.0);
+  p.c = (T)cos(theta);
+  p.s = (T)sin(theta);
+  p.q = (T)cp.q;
+  p.invq = (T)(1.0 / cp.q);
+  p.qm = (T)(cp.q - 1.0 / cp.q);
+  p.soft2 = (T)(cp.softening * cp.softening);
+  p.a = p.inva = p.b = p.db = p.binva = p.inv_s = (T)0;
+  switch (cp.profile) {
+    case APX_SERSIC: {  // prm: n, Re, Ie
+      const double n = cp.prm[0];
+      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[2]));
+      p.a = (T)n;
+      p.inva = (T)(1.0 / n);
+      p.b = (T)sersic_b(n);
+      p.db = (T)sersic_db(n);
+      p.inv_s = (T)(1.0 / cp.prm[1]);  // 1/Re
+      break;
+    }
+    case APX_EXPONENTIAL: {  // prm: Re, Ie ; b(1) from the Sersic polynomial
+      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[1]));
+      p.b = (T)sersic_b(1.0);
+      p.inv_s = (T)(1.0 / cp.prm[0]);
+      break;
+    }
+    case APX_GAUSSIAN: {  // prm: sigma, flux ; I0/sqrt(2 pi sigma^2)
+      const double sg = cp.prm[0];
+      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[1]) / sqrt(2.0 * kPi * sg * sg));
+      p.inv_s = (T)(1.0 / sg);
+      break;
+    }
+    case APX_MOFFAT: {  // prm: n, Rd, I0
+      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[2]));
+      p.a = (T)cp.prm[0];
+      p.inv_s = (T)(1.0 / cp.prm[1]);
+      break;
+    }
+    default: {  // flat sky: prm F
+      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[0]));
+      break;
+    }
+  }
+  return p;
+}
+
+// Value (times `scale`, the pixel-area ratio of the sampling level) and, when
+// GRAD, g[slot] = dI/d(slot value).
+template <typename T, bool GRAD>
+__device__ __forceinline__ T eval_profile(const Prof<T>& p, T X, T Y, T scale, T* g) {
+  const T amp = p.amp * scale;
+  if (p.kind == APX_FLAT_SKY) {
+    if (GRAD) {
+#pragma unroll
+      for (int k = 0; k < APX_NSLOT; ++k) g[k] = (T)0;
+      g[APX_P_A] = amp * (T)kLn10;
+    }
+    return amp;
+  }
+  const T Xr = p.c * X - p.s * Y;
+  const T Yr = (p.s * X + p.c * Y) * p.invq;
+  const T R = f_sqrt(Xr * Xr + Yr * Yr + p.soft2);
+  T I, dIdR = (T)0, dA = (T)0, dB = (T)0, dC = (T)0;
+  switch (p.kind) {
+    case APX_SERSIC: {
+      const T lx = f_log(R * p.inv_s);
+      const T pw = f_exp(lx * p.inva);  // (R/Re)^(1/n)
+      I = amp * f_exp(-p.b * (pw - (T)1));
+      if (GRAD) {
+        const T t = I * p.b * pw * p.inva;
+        dIdR = -t / R;
+        dA = I * (-p.db * (pw - (T)1) + p.b * pw * lx * p.inva * p.inva);
+        dB = t * p.inv_s;
+        dC = I * (T)kLn10;
+      }
+      break;
+    }
+    case APX_EXPONENTIAL: {
+      const T x = R * p.inv_s;
+      I = amp * f_exp(-p.b * (x - (T)1));
+      if (GRAD) {
+        dIdR = -I * p.b * p.inv_s;
+        dA = I * p.b * x * p.inv_s;
+        dB = I * (T)kLn10;
+      }
+      break;
+    }
+    case APX_GAUSSIAN: {
+      const T x = R * p.inv_s;
+      I = amp * f_exp((T)-0.5 * x * x);
+      if (GRAD) {
+        dIdR = -I * x * p.inv_s;
+        dA = I * (x * x - (T)1) * p.inv_s;
+        dB = I * (T)kLn10;
+      }
+      break;
+    }
+    default: {  // APX_MOFFAT
+      const T x = R * p.inv_s;
+      const T u = (T)1 + x * x;
+      const T lu = f_log(u);
+      I = amp * f_exp(-p.a * lu);
+      if (GRAD) {
+        const T t = I * p.a * (T)2 * x / u;  // -dI/dx
+        dIdR = -t * p.inv_s;
+        dA = -I * lu;
+        dB = t * x * p.inv_s;
+        dC = I * (T)kLn10;
+      }
+      break;
+    }
+  }
+  if (GRAD) {
+    const T invR = (T)1 / R;
+    const T dRdX = (p.c * Xr + p.s * Yr * p.invq) * invR;
+    const T dRdY = (-p.s * Xr + p.c * Yr * p.invq) * invR;
+    g[APX_P_CX] = -dIdR * dRdX;  // X = coordinate - centre
+    g[APX_P_CY] = -dIdR * dRdY;
+    g[APX_P_Q] = -dIdR * Yr * Yr * p.invq * invR;
+    g[APX_P_PA] = dIdR * Xr * Yr * p.qm * invR;
+    g[APX_P_A] = dA;
+    g[APX_P_B] = dB;
+    g[APX_P_C] = dC;
+  }
+  return I;
+}
+
+}  // namespace apx

@@ -1,0 +1,236 @@
+// Pixel reductions that feed Levenberg-Marquardt (fit/lm.py:373-406):
+//   chi2             sum W (Y - m)^2 over unmasked pixels    (_chi2)
+//   normal_equations J^T W J and -J^T W (m - Y)                (_hess, _grad)
+//   rpp              J^T ((2/d)(W (m1 - m0)/d - W J h))         (_rpp)
+// Each is a grid-stride pass that writes one float64 partial per workgroup,
+// then a single-workgroup pass sums the partials in a fixed order, so repeated
+// runs give identical bits.  J^T W J uses the gfx950 matrix cores
+// (v_mfma_f32_16x16x4_f32: 4 pixels x 16 parameters per instruction, FP32
+// accumulate, flushed to float64 every 64 instructions).
+#include <hip/hip_runtime.h>
+
+#include "capi_internal.h"
+
+namespace apx {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kBlocks = 512;
+constexpr int kMaxNp = 16;
+
+__host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void chi2_partial_kernel(const float* __restrict__ model,
+                                                           const float* __restrict__ data,
+                                                           const float* __restrict__ weight,
+                                                           const uint8_t* __restrict__ mask,
+                                                           int64_t npix, double* partial) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < npix;
+       p += (int64_t)gridDim.x * 256) {
+    if (mask && mask[p]) continue;
+    const double d = (double)data[p] - (double)model[p];
+    acc += (weight ? (double)weight[p] : 1.0) * d * d;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out[k] = sum_b partial[b * stride + k] for k < n
+__global__ __launch_bounds__(256) void sum_partials_kernel(const double* partial, int nblocks,
+                                                           int stride, int n, double* out,
+                                                           double scale) {
+  for (int k = threadIdx.x; k < n; k += 256) {
+    double t = 0.0;
+    for (int b = 0; b < nblocks; ++b) t += partial[(int64_t)b * stride + k];
+    out[k] = t * scale;
+  }
+}
+
+// J^T W J and J^T W (Y - m): one 16x16x4 MFMA per 4 pixels per wavefront.
+// Lane l holds parameter r = l % 16 of pixel base + l / 16:
+//   A[r][k] = J[p_k][r], B[k][c] = W_p J[p_k][c], D[4(l/16)+i][l%16] (reg i).
+__global__ __launch_bounds__(256) void normal_eq_kernel(const float* __restrict__ J,
+                                                        const float* __restrict__ model,
+                                                        const float* __restrict__ data,
+                                                        const float* __restrict__ weight,
+                                                        const uint8_t* __restrict__ mask,
+                                                        int64_t npix, int np, double* partial) {
+  __shared__ double redH[4][256];
+  __shared__ double redg[4][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = lane & 15, k = lane >> 4;
+  const int64_t gwave = (int64_t)blockIdx.x * 4 + wv;
+  const int64_t nwave = (int64_t)gridDim.x * 4;
+  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  double accd[4] = {0.0, 0.0, 0.0, 0.0};
+  double gacc = 0.0;
+  float gpart = 0.0f;
+  int since = 0;
+  for (int64_t base = gwave * 4; base < npix; base += nwave * 4) {
+    const int64_t p = base + k;
+    float a = 0.0f, w = 0.0f, res = 0.0f;
+    if (p < npix && r < np && !(mask && mask[p])) {
+      a = J[p * np + r];
+      w = weight ? weight[p] : 1.0f;
+      res = data[p] - model[p];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, w * a, acc, 0, 0, 0);
+    gpart = fmaf(a * w, res, gpart);
+    if (++since == 64) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        accd[i] += (double)acc[i];
+        acc[i] = 0.0f;
+      }
+      gacc += (double)gpart;
+      gpart = 0.0f;
+      since = 0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) accd[i] += (double)acc[i];
+  gacc += (double)gpart;
+  // H element of (lane, reg i): row 4*(lane/16)+i, col lane%16
+#pragma unroll
+  for (int i = 0; i < 4; ++i) redH[wv][(4 * (lane >> 4) + i) * 16 + (lane & 15)] = accd[i];
+  gacc += __shfl_xor(gacc, 16, 64);
+  gacc += __shfl_xor(gacc, 32, 64);
+  if (lane < 16) redg[wv][lane] = gacc;
+  __syncthreads();
+  const int stride = kMaxNp * kMaxNp + kMaxNp;
+  double* out = partial + (int64_t)blockIdx.x * stride;
+  for (int e = threadIdx.x; e < 256; e += 256)
+    out[e] = redH[0][e] + redH[1][e] + redH[2][e] + redH[3][e];
+  if (threadIdx.x < 16)
+    out[256 + threadIdx.x] = redg[0][threadIdx.x] + redg[1][threadIdx.x] + redg[2][threadIdx.x] +
+                             redg[3][threadIdx.x];
+}
+
+// unpack 16x16 padded partial sums into np x np hess and np grad
+__global__ __launch_bounds__(256) void normal_eq_final_kernel(const double* partial, int nblocks,
+                                                              int np, double* hess, double* grad) {
+  const int stride = kMaxNp * kMaxNp + kMaxNp;
+  for (int e = threadIdx.x; e < np * np + np; e += 256) {
+    const int src = e < np * np ? (e / np) * kMaxNp + (e % np) : 256 + (e - np * np);
+    double t = 0.0;
+    for (int b = 0; b < nblocks; ++b) t += partial[(int64_t)b * stride + src];
+    if (e < np * np)
+      hess[e] = t;
+    else
+      grad[e - np * np] = t;
+  }
+}
+
+__global__ __launch_bounds__(256) void rpp_kernel(const float* __restrict__ J,
+                                                  const float* __restrict__ m0,
+                                                  const float* __restrict__ m1,
+                                                  const float* __restrict__ weight,
+                                                  const uint8_t* __restrict__ mask, int64_t npix,
+                                                  int np, const double* __restrict__ h, double d,
+                                                  double* partial) {
+  __shared__ double red[4][kMaxNp];
+  double hv[kMaxNp], acc[kMaxNp];
+#pragma unroll
+  for (int r = 0; r < kMaxNp; ++r) {
+    hv[r] = r < np ? h[r] : 0.0;
+    acc[r] = 0.0;
+  }
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < npix;
+       p += (int64_t)gridDim.x * 256) {
+    if (mask && mask[p]) continue;
+    float jr[kMaxNp];
+    double jh = 0.0;
+#pragma unroll
+    for (int r = 0; r < kMaxNp; ++r) {
+      jr[r] = r < np ? J[p * np + r] : 0.0f;
+      jh += (double)jr[r] * hv[r];
+    }
+    const double w = weight ? (double)weight[p] : 1.0;
+    const double v = (2.0 / d) * (w * (double)(m1[p] - m0[p]) / d - w * jh);
+#pragma unroll
+    for (int r = 0; r < kMaxNp; ++r) acc[r] += (double)jr[r] * v;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < kMaxNp; ++r) {
+    const double t = wave_sum(acc[r]);
+    if (lane == 0) red[wv][r] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMaxNp)
+    partial[(int64_t)blockIdx.x * kMaxNp + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+int nblocks_for(int64_t npix) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kBlocks, (npix + 1023) / 1024));
+}
+
+}  // namespace
+
+int64_t reduce_workspace_bytes(int64_t npix, int32_t np) {
+  (void)np;
+  return align256(8 * (int64_t)kBlocks * (kMaxNp * kMaxNp + kMaxNp));
+}
+
+int chi2(const float* model, const float* data, const float* weight, const uint8_t* mask,
+         int64_t npix, double* out, void* ws, int64_t ws_bytes, hipStream_t stream) {
+  if (!model || !data || !out || !ws) return set_error(APX_E_INVALID, "chi2: null pointer");
+  if (npix < 0) return set_error(APX_E_INVALID, "chi2: negative size");
+  if (ws_bytes < reduce_workspace_bytes(npix, 1))
+    return set_error(APX_E_INVALID, "chi2: workspace too small");
+  const int nb = nblocks_for(npix);
+  double* partial = static_cast<double*>(ws);
+  hipLaunchKernelGGL(chi2_partial_kernel, dim3(nb), dim3(256), 0, stream, model, data, weight,
+                     mask, npix, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, stream, partial, nb, 1, 1, out,
+                     1.0);
+  return check_launch("chi2");
+}
+
+int normal_equations(const float* J, const float* model, const float* data, const float* weight,
+                     const uint8_t* mask, int64_t npix, int32_t np, double* hess, double* grad,
+                     void* ws, int64_t ws_bytes, hipStream_t stream) {
+  if (!J || !model || !data || !hess || !grad || !ws)
+    return set_error(APX_E_INVALID, "normal_equations: null pointer");
+  if (np < 1 || np > kMaxNp) return set_error(APX_E_LIMIT, "normal_equations: np must be 1..16");
+  if (ws_bytes < reduce_workspace_bytes(npix, np))
+    return set_error(APX_E_INVALID, "normal_equations: workspace too small");
+  const int nb = nblocks_for(npix);
+  double* partial = static_cast<double*>(ws);
+  hipLaunchKernelGGL(normal_eq_kernel, dim3(nb), dim3(256), 0, stream, J, model, data, weight,
+                     mask, npix, np, partial);
+  hipLaunchKernelGGL(normal_eq_final_kernel, dim3(1), dim3(256), 0, stream, partial, nb, np, hess,
+                     grad);
+  return check_launch("normal_equations");
+}
+
+int rpp(const float* J, const float* m0, const float* m1, const float* weight, const uint8_t* mask,
+        int64_t npix, int32_t np, const double* h, double d, double* out, void* ws,
+        int64_t ws_bytes, hipStream_t stream) {
+  if (!J || !m0 || !m1 || !h || !out || !ws) return set_error(APX_E_INVALID, "rpp: null pointer");
+  if (np < 1 || np > kMaxNp) return set_error(APX_E_LIMIT, "rpp: np must be 1..16");
+  if (ws_bytes < reduce_workspace_bytes(npix, np))
+    return set_error(APX_E_INVALID, "rpp: workspace too small");
+  const int nb = nblocks_for(npix);
+  double* partial = static_cast<double*>(ws);
+  hipLaunchKernelGGL(rpp_kernel, dim3(nb), dim3(256), 0, stream, J, m0, m1, weight, mask, npix, np,
+                     h, d, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, stream, partial, nb, kMaxNp, np,
+                     out, 1.0);
+  return check_launch("rpp");
+}
+
+}  // namespace apx

@@ -1,0 +1,607 @@
+"""Windows and images with the reference's API (image/window_object.py,
+image/wcs.py PPCS, image/image_object.py, image/target_image.py,
+image/psf_image.py, image/model_image.py, image/jacobian_image.py).
+
+Window geometry is host-side float64, exactly as the reference computes it;
+pixel data live on the GPU in float32 (``AP_config.ap_image_dtype``).  World
+(RA/Dec) coordinates and FITS I/O are not part of this package.
+"""
+
+from __future__ import annotations
+
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import AP_config
+from .errors import InvalidData, SpecificationConflict
+
+__all__ = ["Window", "Image", "Target_Image", "Model_Image", "PSF_Image", "Jacobian_Image"]
+
+F64 = torch.float64
+
+
+def _h(v):
+    """host float64 tensor"""
+    return torch.as_tensor(v, dtype=F64, device="cpu")
+
+
+def _dev(v, dtype=None):
+    dtype = AP_config.ap_image_dtype if dtype is None else dtype
+    if isinstance(v, torch.Tensor):
+        return v.detach().to(device=AP_config.ap_device, dtype=dtype)
+    return torch.as_tensor(np.asarray(v), dtype=dtype, device=AP_config.ap_device)
+
+
+class Window:
+    """A window on the tangent plane: ``pixel_shape`` (nx, ny) pixels of
+    parallelogram ``pixelscale``; pixel ``reference_imageij`` is pinned to plane
+    point ``reference_imagexy`` (image/wcs.py:561-584)."""
+
+    def __init__(self, *, pixel_shape=None, origin=None, center=None, pixelscale=None,
+                 reference_imageij=(-0.5, -0.5), reference_imagexy=(0.0, 0.0), state=None, **kwargs):
+        if state is not None:
+            self.set_state(state)
+            return
+        self.pixelscale = 1.0 if pixelscale is None else pixelscale
+        self.pixel_shape = pixel_shape
+        if origin is not None and center is not None:
+            raise SpecificationConflict("Please provide only one reference position for the window")
+        if origin is not None:  # window_object.py:143-150
+            self.reference_imageij = (-0.5, -0.5)
+            self.reference_imagexy = origin
+        elif center is not None:  # window_object.py:151-160
+            self.reference_imageij = self.pixel_shape.to(F64) / 2 - 0.5
+            self.reference_imagexy = center
+        else:
+            self.reference_imageij = reference_imageij
+            self.reference_imagexy = reference_imagexy
+
+    # -- PPCS (wcs.py:492-639) ----------------------------------------------
+    @property
+    def pixelscale(self):
+        return self._pixelscale
+
+    @pixelscale.setter
+    def pixelscale(self, pix):
+        p = _h(pix).clone()
+        if p.numel() == 1:
+            p = torch.eye(2, dtype=F64) * p.reshape(())
+        self._pixelscale = p.reshape(2, 2)
+        self._pixel_area = torch.linalg.det(self._pixelscale).abs()
+        self._pixelscale_inv = torch.linalg.inv(self._pixelscale)
+
+    @property
+    def pixel_area(self):
+        return self._pixel_area
+
+    @property
+    def pixel_length(self):
+        return self._pixel_area.sqrt()
+
+    @property
+    def pixelscale_inv(self):
+        return self._pixelscale_inv
+
+    @property
+    def reference_imageij(self):
+        return self._reference_imageij
+
+    @reference_imageij.setter
+    def reference_imageij(self, v):
+        self._reference_imageij = _h(v).clone()
+
+    @property
+    def reference_imagexy(self):
+        return self._reference_imagexy
+
+    @reference_imagexy.setter
+    def reference_imagexy(self, v):
+        self._reference_imagexy = _h(v).clone()
+
+    @property
+    def pixel_shape(self):
+        return self._pixel_shape
+
+    @pixel_shape.setter
+    def pixel_shape(self, shape):
+        self._pixel_shape = None if shape is None else torch.round(_h(shape)).to(torch.int32)
+
+    def pixel_to_plane(self, ij):
+        return self.pixelscale @ (_h(ij) - self.reference_imageij) + self.reference_imagexy
+
+    def plane_to_pixel(self, xy):
+        return self.pixelscale_inv @ (_h(xy) - self.reference_imagexy) + self.reference_imageij
+
+    def pixel_to_plane_delta(self, d):
+        return self.pixelscale @ _h(d)
+
+    def plane_to_pixel_delta(self, d):
+        return self.pixelscale_inv @ _h(d)
+
+    # -- window properties (window_object.py:165-239) ---------------------
+    @property
+    def nx(self):
+        return int(self.pixel_shape[0])
+
+    @property
+    def ny(self):
+        return int(self.pixel_shape[1])
+
+    @property
+    def size(self):
+        return self.nx * self.ny
+
+    @property
+    def end(self):
+        return self.pixel_to_plane_delta(self.pixel_shape.to(F64))
+
+    @property
+    def origin(self):
+        return self.pixel_to_plane(-0.5 * torch.ones(2, dtype=F64))
+
+    @property
+    def center(self):
+        return self.origin + self.end / 2
+
+    @property
+    def shape(self):
+        S1 = self.pixel_shape.to(F64).clone()
+        S1[1] = 0.0
+        S2 = self.pixel_shape.to(F64).clone()
+        S2[0] = 0.0
+        return torch.stack((torch.linalg.norm(self.pixelscale @ S1), torch.linalg.norm(self.pixelscale @ S2)))
+
+    def copy(self, **kwargs):
+        w = Window.__new__(Window)
+        w._pixelscale = self._pixelscale.clone()
+        w._pixel_area = self._pixel_area.clone()
+        w._pixelscale_inv = self._pixelscale_inv.clone()
+        w._pixel_shape = self._pixel_shape.clone()
+        w._reference_imageij = self._reference_imageij.clone()
+        w._reference_imagexy = self._reference_imagexy.clone()
+        if "pixelscale" in kwargs:
+            w.pixelscale = kwargs["pixelscale"]
+        if "pixel_shape" in kwargs:
+            w.pixel_shape = kwargs["pixel_shape"]
+        if "origin" in kwargs:
+            w.reference_imageij = (-0.5, -0.5)
+            w.reference_imagexy = kwargs["origin"]
+        if "reference_imageij" in kwargs:
+            w.reference_imageij = kwargs["reference_imageij"]
+        if "reference_imagexy" in kwargs:
+            w.reference_imagexy = kwargs["reference_imagexy"]
+        return w
+
+    def rescale_pixel(self, scale):
+        """window_object.py:233-239."""
+        return self.copy(pixelscale=self.pixelscale * scale,
+                         pixel_shape=self.pixel_shape.to(F64) // scale,
+                         reference_imageij=(self.reference_imageij + 0.5) / scale - 0.5)
+
+    def pixel_shift(self, shift):
+        """window_object.py:284-290."""
+        self.reference_imageij = self.reference_imageij - _h(shift)
+        return self
+
+    def shift(self, shift):
+        self.reference_imagexy = self.reference_imagexy + _h(shift)
+        return self
+
+    def _pix(self, pixels):
+        pixels = [int(p) if not isinstance(p, torch.Tensor) else p for p in pixels]
+        return _h([float(p) for p in pixels])
+
+    def pad_pixel(self, pixels):
+        """window_object.py:367-388."""
+        p = self._pix(pixels)
+        if len(p) == 1:
+            p = p.repeat(2)
+        if len(p) == 2:
+            self.pixel_shape = self.pixel_shape.to(F64) + 2 * p
+            self.reference_imageij = self.reference_imageij + p
+        elif len(p) == 4:
+            self.pixel_shape = self.pixel_shape.to(F64) + p[:2] + p[2:]
+            self.reference_imageij = self.reference_imageij + p[:2]
+        else:
+            raise ValueError(f"Unrecognized pixel pad format: {pixels}")
+        return self
+
+    def crop_pixel(self, pixels):
+        """window_object.py:335-356."""
+        p = self._pix(pixels)
+        if len(p) == 1:
+            p = p.repeat(2)
+        if len(p) == 2:
+            self.pixel_shape = self.pixel_shape.to(F64) - 2 * p
+            self.reference_imageij = self.reference_imageij - p
+        elif len(p) == 4:
+            self.pixel_shape = self.pixel_shape.to(F64) - p[:2] - p[2:]
+            self.reference_imageij = self.reference_imageij - p[:2]
+        else:
+            raise ValueError(f"Unrecognized pixel crop format: {pixels}")
+        return self
+
+    def crop_to_pixel(self, pixels):
+        """window_object.py:358-365; pixels = [[xmin, xmax], [ymin, ymax]]."""
+        p = _h(pixels)
+        self.reference_imageij = self.reference_imageij - p[:, 0]
+        self.pixel_shape = p[:, 1] - p[:, 0]
+        return self
+
+    @staticmethod
+    def _get_indices(ref_window, obj_window):
+        """window_object.py:241-253: (row slice, column slice) of ref's pixels
+        covered by obj."""
+        o = torch.round(ref_window.plane_to_pixel(obj_window.origin) + 0.5).int()
+        o = torch.maximum(torch.zeros_like(o), o)
+        e = torch.round(ref_window.plane_to_pixel(obj_window.origin + obj_window.end) + 0.5).int()
+        e = torch.minimum(ref_window.pixel_shape, e)
+        return slice(int(o[1]), int(e[1])), slice(int(o[0]), int(e[0]))
+
+    def get_self_indices(self, obj):
+        return self._get_indices(self, obj if isinstance(obj, Window) else obj.window)
+
+    def get_other_indices(self, obj):
+        return self._get_indices(obj if isinstance(obj, Window) else obj.window, self)
+
+    def __and__(self, other):
+        """window_object.py:511-523."""
+        o = self.plane_to_pixel(other.origin)
+        lo = torch.maximum(-0.5 * torch.ones_like(o), o)
+        e = self.plane_to_pixel(other.origin + other.end)
+        hi = torch.minimum(self.pixel_shape.to(F64) - 0.5, e)
+        return self.copy(origin=self.pixel_to_plane(lo), pixel_shape=hi - lo)
+
+    def __or__(self, other):
+        """window_object.py:483-495."""
+        o = self.plane_to_pixel(other.origin)
+        lo = torch.minimum(-0.5 * torch.ones_like(o), o)
+        e = self.plane_to_pixel(other.origin + other.end)
+        hi = torch.maximum(self.pixel_shape.to(F64), e)
+        return self.copy(origin=self.pixel_to_plane(lo), pixel_shape=hi - lo)
+
+    def overlap_frac(self, other):
+        overlap = self & other
+        oa = torch.prod(overlap.shape)
+        return oa / (torch.prod(self.shape) + torch.prod(other.shape) - oa)
+
+    def get_coordinate_meshgrid(self, device="cpu"):
+        """window_object.py:390-405: plane coordinates of every pixel centre (2, ny, nx)."""
+        i = torch.arange(self.nx, dtype=F64)
+        j = torch.arange(self.ny, dtype=F64)
+        mi, mj = torch.meshgrid(i, j, indexing="xy")
+        c = self.pixelscale @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.reference_imageij.view(2, 1))
+        return (c + self.reference_imagexy.view(2, 1)).reshape(2, self.ny, self.nx).to(device)
+
+    def __eq__(self, other):
+        return (torch.all(self.pixel_shape == other.pixel_shape).item()
+                and torch.all(self.pixelscale == other.pixelscale).item()
+                and torch.all(self.pixel_to_plane(torch.zeros(2)) == other.pixel_to_plane(torch.zeros(2))).item())
+
+    def get_state(self):
+        return {"pixel_shape": self.pixel_shape.tolist(), "pixelscale": self.pixelscale.tolist(),
+                "reference_imageij": self.reference_imageij.tolist(),
+                "reference_imagexy": self.reference_imagexy.tolist()}
+
+    def set_state(self, state):
+        self.pixelscale = state.get("pixelscale", 1.0)
+        self.pixel_shape = state["pixel_shape"]
+        self.reference_imageij = state.get("reference_imageij", (-0.5, -0.5))
+        self.reference_imagexy = state.get("reference_imagexy", (0.0, 0.0))
+
+    def __repr__(self):
+        return (f"window pixel_shape: {self.pixel_shape.tolist()}, origin: {self.origin.tolist()}, "
+                f"pixelscale: {self.pixelscale.tolist()}")
+
+
+class Image:
+    """Pixel data on the GPU plus a window (image/image_object.py:17-479)."""
+
+    north = math.pi / 2  # image/image_header.py:36
+
+    def __init__(self, *, data=None, pixelscale=None, window: Optional[Window] = None, zeropoint=None,
+                 origin=None, center=None, identity=None, **kwargs):
+        if data is None and window is None:
+            raise InvalidData("Image must have either data or a window to construct itself.")
+        if window is None:
+            shape = tuple(np.shape(data)) if not isinstance(data, torch.Tensor) else tuple(data.shape)
+            window = Window(pixel_shape=(shape[1], shape[0]), pixelscale=pixelscale, origin=origin,
+                            center=center)
+        self.window = window
+        self.zeropoint = None if zeropoint is None else _h(zeropoint)
+        self.identity = str(id(self)) if identity is None else identity
+        if data is None:
+            self._data = torch.zeros((window.ny, window.nx), dtype=AP_config.ap_image_dtype,
+                                     device=AP_config.ap_device)
+        else:
+            self.set_data(data, require_shape=False)
+
+    @property
+    def data(self):
+        return self._data
+
+    @data.setter
+    def data(self, d):
+        self.set_data(d)
+
+    def set_data(self, data, require_shape=True):
+        d = _dev(data)
+        if require_shape and getattr(self, "_data", None) is not None and d.shape != self._data.shape:
+            raise SpecificationConflict(
+                f"Attempting to change image data with tensor that has a different shape! "
+                f"({tuple(d.shape)} vs {tuple(self._data.shape)})")
+        self._data = d
+
+    @property
+    def pixelscale(self):
+        return self.window.pixelscale
+
+    @property
+    def pixel_area(self):
+        return self.window.pixel_area
+
+    @property
+    def pixel_length(self):
+        return self.window.pixel_length
+
+    @property
+    def origin(self):
+        return self.window.origin
+
+    @property
+    def center(self):
+        return self.window.center
+
+    @property
+    def shape(self):
+        return self.window.shape
+
+    def plane_to_pixel(self, xy):
+        return self.window.plane_to_pixel(xy)
+
+    def pixel_to_plane(self, ij):
+        return self.window.pixel_to_plane(ij)
+
+    def get_coordinate_meshgrid(self):
+        return self.window.get_coordinate_meshgrid()
+
+    def _new(self, data, window):
+        return self.__class__(data=data, window=window, zeropoint=self.zeropoint, identity=self.identity)
+
+    def copy(self):
+        return self._new(self.data.clone(), self.window.copy())
+
+    def blank_copy(self):
+        return self._new(torch.zeros_like(self.data), self.window.copy())
+
+    def get_window(self, window: Window):
+        """image_object.py:294-300 (sub-image covering ``window``)."""
+        return self._new(self.data[self.window.get_self_indices(window)], self.window & window)
+
+    def __getitem__(self, key):
+        if isinstance(key, Window):
+            return self.get_window(key)
+        if isinstance(key, Image):
+            return self.get_window(key.window)
+        raise ValueError("Unrecognized Image getitem request!")
+
+    def flatten(self, attribute="data"):
+        return getattr(self, attribute).reshape(-1)
+
+    def crop(self, pixels):
+        """image_object.py:307-334."""
+        p = [int(v) for v in pixels]
+        if len(p) == 1:
+            self.set_data(self.data[p[0]:self.data.shape[0] - p[0], p[0]:self.data.shape[1] - p[0]],
+                          require_shape=False)
+        elif len(p) == 2:
+            self.set_data(self.data[p[1]:self.data.shape[0] - p[1], p[0]:self.data.shape[1] - p[0]],
+                          require_shape=False)
+        elif len(p) == 4:
+            self.set_data(self.data[p[2]:self.data.shape[0] - p[3], p[0]:self.data.shape[1] - p[1]],
+                          require_shape=False)
+        self.window.crop_pixel(p)
+        return self
+
+    def reduce(self, scale: int):
+        """image_object.py:348-376: sum scale x scale blocks."""
+        if not isinstance(scale, int):
+            raise SpecificationConflict(f"Reduce scale must be an integer! not {type(scale)}")
+        if scale == 1:
+            return self
+        MS, NS = self.data.shape[0] // scale, self.data.shape[1] // scale
+        d = self.data[: MS * scale, : NS * scale].reshape(MS, scale, NS, scale).sum(dim=(1, 3))
+        return self._new(d, self.window.rescale_pixel(scale))
+
+    def __iadd__(self, other):
+        """image_object.py:450-457."""
+        if isinstance(other, Image):
+            self.data[other.window.get_other_indices(self)] += other.data[self.window.get_other_indices(other)]
+        else:
+            self.data += other
+        return self
+
+    def __isub__(self, other):
+        if isinstance(other, Image):
+            self.data[other.window.get_other_indices(self)] -= other.data[self.window.get_other_indices(other)]
+        else:
+            self.data -= other
+        return self
+
+    def __repr__(self):
+        return f"{self.__class__.__name__} {tuple(self.data.shape)} {self.window!r}"
+
+
+class Model_Image(Image):
+    """image/model_image.py:13-91."""
+
+    def clear_image(self):
+        self.data.zero_()
+
+
+class PSF_Image(Image):
+    """image/psf_image.py: odd-shaped PSF pinned at its centre pixel."""
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.window.reference_imageij = np.flip(np.array(self.data.shape, dtype=float) - 1.0).copy() / 2
+        self.window.reference_imagexy = (0.0, 0.0)
+
+    def set_data(self, data, require_shape=True):
+        super().set_data(data, require_shape)
+        if any(s % 2 != 1 for s in self.data.shape):
+            raise SpecificationConflict(f"psf must have odd shape, not {tuple(self.data.shape)}")
+        if torch.any(self.data < 0):
+            AP_config.ap_logger.warning("psf data should be non-negative")
+
+    def normalize(self):
+        self.data /= torch.sum(self.data)
+
+    @property
+    def psf_border_int(self):
+        """psf_image.py:71-93: ceil((1 + shape)/2) in (x, y) order."""
+        return (int(math.ceil((1 + self.data.shape[1]) / 2)), int(math.ceil((1 + self.data.shape[0]) / 2)))
+
+    @property
+    def mask(self):
+        return torch.zeros_like(self.data, dtype=torch.bool)
+
+
+class Jacobian_Image(Image):
+    """image/jacobian_image.py: (ny, nx, Np) float32, pixel-major; parameter
+    identities label the last axis."""
+
+    def __init__(self, *, parameters, data=None, window: Window, **kwargs):
+        self.parameters = list(parameters)
+        if data is None:
+            data = torch.zeros((window.ny, window.nx, len(self.parameters)),
+                               dtype=AP_config.ap_image_dtype, device=AP_config.ap_device)
+        super().__init__(data=data, window=window, **kwargs)
+
+    def set_data(self, data, require_shape=True):
+        self._data = _dev(data)
+
+    def flatten(self, attribute="data"):
+        """jacobian_image.py:36-37: (ny*nx, Np)."""
+        d = getattr(self, attribute)
+        return d.reshape(-1, d.shape[-1])
+
+    def _new(self, data, window):
+        return Jacobian_Image(parameters=self.parameters, data=data, window=window)
+
+    def __iadd__(self, other):
+        """jacobian_image.py:79-118: add columns matched by parameter identity."""
+        if not isinstance(other, Jacobian_Image):
+            raise ValueError("Jacobian images can only add Jacobian images")
+        si = self.window.get_other_indices(other)
+        oi = other.window.get_other_indices(self)
+        for k, pid in enumerate(other.parameters):
+            if pid in self.parameters:
+                j = self.parameters.index(pid)
+                self.data[oi + (j,)] += other.data[si + (k,)]
+            else:
+                self.parameters.append(pid)
+                col = torch.zeros(self.data.shape[:2] + (1,), dtype=self.data.dtype, device=self.data.device)
+                col[oi + (0,)] = other.data[si + (k,)]
+                self._data = torch.cat((self.data, col), dim=-1)
+        return self
+
+
+class Target_Image(Image):
+    """image/target_image.py:17-532: data, variance/weight, mask and PSF."""
+
+    def __init__(self, *args, mask=None, variance=None, weight=None, psf=None, **kwargs):
+        super().__init__(*args, **kwargs)
+        self._weight = None
+        self._mask = None
+        self._psf = None
+        if weight is not None:
+            self.weight = weight
+        elif variance is not None:
+            self.variance = variance
+        if mask is not None:
+            self.mask = mask
+        if psf is not None:
+            self.psf = psf
+
+    # variance / weight (target_image.py:117-184, 271-281)
+    @property
+    def has_variance(self):
+        return self._weight is not None
+
+    @property
+    def weight(self):
+        if self._weight is None:
+            return torch.ones_like(self.data)
+        return self._weight
+
+    @weight.setter
+    def weight(self, w):
+        self._weight = None if w is None else _dev(w)
+
+    @property
+    def variance(self):
+        if self._weight is None:
+            return torch.ones_like(self.data)
+        return 1.0 / self._weight
+
+    @variance.setter
+    def variance(self, v):
+        self._weight = None if v is None else 1.0 / _dev(v, torch.float64).to(AP_config.ap_image_dtype)
+
+    @property
+    def has_mask(self):
+        return self._mask is not None
+
+    @property
+    def mask(self):
+        if self._mask is None:
+            return torch.zeros_like(self.data, dtype=torch.bool)
+        return self._mask
+
+    @mask.setter
+    def mask(self, m):
+        self._mask = None if m is None else _dev(m, torch.bool)
+
+    @property
+    def has_psf(self):
+        return self._psf is not None
+
+    @property
+    def psf(self):
+        return self._psf
+
+    @psf.setter
+    def psf(self, psf):
+        """target_image.py:303-325."""
+        if psf is None or isinstance(psf, PSF_Image):
+            self._psf = psf
+        else:
+            self._psf = PSF_Image(data=psf, pixelscale=self.pixelscale)
+
+    def _new(self, data, window):
+        return Target_Image(data=data, window=window, zeropoint=self.zeropoint, identity=self.identity)
+
+    def get_window(self, window):
+        idx = self.window.get_self_indices(window)
+        new = Target_Image(data=self.data[idx], window=self.window & window, zeropoint=self.zeropoint,
+                           identity=self.identity)
+        if self._weight is not None:
+            new._weight = self._weight[idx]
+        if self._mask is not None:
+            new._mask = self._mask[idx]
+        new._psf = self._psf
+        return new
+
+    def jacobian_image(self, parameters=None, data=None):
+        """target_image.py:406-430."""
+        return Jacobian_Image(parameters=[] if parameters is None else parameters, data=data,
+                              window=self.window.copy())
+
+    def model_image(self, window=None):
+        """target_image.py:432-441."""
+        w = self.window.copy() if window is None else window.copy()
+        return Model_Image(window=w)

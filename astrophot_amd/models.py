@@ -1,0 +1,466 @@
+"""Model classes with the reference's API (models/core_model.py,
+model_object.py, galaxy_model_object.py, sersic_model.py, exponential_model.py,
+gaussian_model.py, moffat_model.py, sky_model_object.py, flatsky_model.py,
+group_model_object.py).
+
+``AstroPhot_Model(model_type=...)`` dispatches on the ``model_type`` string
+(core_model.py:98-115).  ``sample`` and ``jacobian`` run on the gfx950
+kernels (see ``_engine``); the class attributes are the reference's knobs
+(model_object.py:63-111) and keep its defaults.
+"""
+
+from __future__ import annotations
+
+import math
+from collections import OrderedDict
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import AP_config
+from . import _engine
+from . import _lib as L
+from .errors import InvalidTarget, SpecificationConflict, UnrecognizedModel
+from .image import Jacobian_Image, Model_Image, PSF_Image, Target_Image, Window
+from .param import Param_Mask, Parameter_Node
+
+__all__ = ["AstroPhot_Model", "Component_Model", "Galaxy_Model", "Sersic_Galaxy", "Exponential_Galaxy",
+           "Gaussian_Galaxy", "Moffat_Galaxy", "Sky_Model", "Flat_Sky", "Group_Model"]
+
+
+def _all_subclasses(cls):
+    out = []
+    for s in cls.__subclasses__():
+        out.append(s)
+        out.extend(_all_subclasses(s))
+    return out
+
+
+class AstroPhot_Model:
+    """core_model.py:55-502."""
+
+    model_type = "model"
+    usable = False
+    default_uncertainty = 1e-2
+
+    def __new__(cls, *, model_type=None, **kwargs):
+        if model_type is not None:
+            for M in _all_subclasses(AstroPhot_Model):
+                if M.model_type == model_type and M.usable:
+                    return super().__new__(M)
+            raise UnrecognizedModel(f"Unknown AstroPhot model type: {model_type}")
+        return super().__new__(cls)
+
+    def __init__(self, *, name=None, target=None, window=None, locked=False, **kwargs):
+        self.name = name if name is not None else f"{self.model_type}:{id(self)}"
+        self.parameters = Parameter_Node(self.name)
+        self._target = None
+        self._window = None
+        self.target = target
+        self.window = window
+        self.locked = locked
+        self.mask = kwargs.get("mask", None)
+
+    @classmethod
+    def List_Models(cls, usable=None):
+        out = _all_subclasses(AstroPhot_Model)
+        return [m for m in out if usable is None or m.usable == usable]
+
+    @property
+    def target(self):
+        return self._target
+
+    @target.setter
+    def target(self, tar):
+        if not (tar is None or isinstance(tar, Target_Image)):
+            raise InvalidTarget("AstroPhot_Model target must be a Target_Image instance.")
+        self._target = tar
+
+    @property
+    def window(self):
+        """core_model.py:291-309."""
+        if self._window is None:
+            if self.target is None:
+                raise ValueError("This model has no target or window, these must be provided by the user")
+            return self.target.window.copy()
+        return self._window
+
+    @window.setter
+    def window(self, window):
+        """core_model.py:311-326."""
+        if window is None:
+            self._window = None
+        elif isinstance(window, Window):
+            self._window = window
+        elif len(window) == 2:
+            self._window = self.target.window.copy().crop_to_pixel(window)
+        else:
+            raise SpecificationConflict(f"Unrecognized window format: {window}")
+
+    def make_model_image(self, window: Optional[Window] = None):
+        """core_model.py:193-204."""
+        w = self.window if window is None else (self.window & window)
+        return self.target[w].model_image()
+
+    def fit_mask(self):
+        return torch.zeros_like(self.target[self.window].mask)
+
+    def __call__(self, image=None, parameters=None, window=None, as_representation=False, **kwargs):
+        """core_model.py:484-502."""
+        if parameters is None:
+            parameters = self.parameters
+        elif isinstance(parameters, torch.Tensor):
+            if as_representation:
+                self.parameters.vector_set_representation(parameters)
+            else:
+                self.parameters.vector_set_values(parameters)
+            parameters = self.parameters
+        return self.sample(image=image, window=window, parameters=parameters, **kwargs)
+
+    def __getitem__(self, key):
+        return self.parameters[key]
+
+    def __contains__(self, key):
+        return key in self.parameters
+
+    def initialize(self, target=None, parameters=None, **kwargs):
+        pass
+
+
+class Component_Model(AstroPhot_Model):
+    """model_object.py:25-444."""
+
+    parameter_specs = {"center": {"units": "arcsec", "uncertainty": [0.1, 0.1]}}
+    _parameter_order = ("center",)
+    _profile = None
+    psf_mode = "none"
+    psf_convolve_mode = "fft"
+    psf_subpixel_shift = "bilinear"
+    sampling_mode = "midpoint"
+    sampling_tolerance = 1e-2
+    integrate_mode = "threshold"
+    integrate_max_depth = 3
+    integrate_gridding = 5
+    integrate_quad_level = 3
+    jacobian_chunksize = 10
+    image_chunksize = 1000
+    softening = 1e-3
+    track_attrs = ["psf_mode", "psf_convolve_mode", "psf_subpixel_shift", "sampling_mode",
+                   "sampling_tolerance", "integrate_mode", "integrate_max_depth", "integrate_gridding",
+                   "integrate_quad_level", "jacobian_chunksize", "image_chunksize", "softening"]
+
+    def __init__(self, *, name=None, psf=None, parameters=None, **kwargs):
+        super().__init__(name=name, **kwargs)
+        self._psf = None
+        for key in self.track_attrs:
+            if key in kwargs:
+                setattr(self, key, kwargs[key])
+        if psf is not None:
+            self.psf = psf
+        specs = self.build_parameter_specs(parameters)
+        for p in self._parameter_order:
+            self.parameters.link(Parameter_Node(p, **specs[p]))
+        if isinstance(parameters, torch.Tensor):
+            self.parameters.vector_set_values(parameters)
+
+    @classmethod
+    def build_parameter_specs(cls, user_specs=None):
+        """_model_methods.py:43-65."""
+        specs = {}
+        for base in reversed(cls.__mro__):
+            if "parameter_specs" in base.__dict__:
+                for k, v in base.__dict__["parameter_specs"].items():
+                    specs[k] = dict(v)
+        if isinstance(user_specs, dict):
+            for p, v in user_specs.items():
+                if isinstance(v, dict):
+                    specs[p].update(v)
+                else:
+                    specs[p]["value"] = v
+        return specs
+
+    @property
+    def psf(self):
+        if self._psf is None:
+            return None if self.target is None else self.target.psf
+        return self._psf
+
+    @psf.setter
+    def psf(self, val):
+        if val is None or isinstance(val, PSF_Image):
+            self._psf = val
+        else:
+            self._psf = PSF_Image(data=val, pixelscale=self.target.pixelscale)
+
+    # -- parameter bookkeeping for the engine ------------------------------
+    def _values(self):
+        return {k: n.value for k, n in self.parameters.nodes.items()}
+
+    def _slot_of(self, name, element):
+        if name == "center":
+            return L.APX_P_CX + element
+        if name in _engine.GEOM_SLOTS:
+            return _engine.GEOM_SLOTS[name]
+        return _engine.PROFILE_SLOTS[self._profile][name]
+
+    def _free_slots(self):
+        """(slot, dval/drep) for every element of the flat parameter vector."""
+        out = []
+        for leaf in self.parameters._leaves():
+            rep = leaf._leaf_val_to_rep(leaf.value)
+            d = leaf._leaf_dval_drep(rep).reshape(-1)
+            for e in range(leaf.size):
+                if bool(leaf.mask.reshape(-1)[e]):
+                    out.append((self._slot_of(leaf.name, e), float(d[e])))
+        return out
+
+    # -- sampling (model_object.py:258-375) ---------------------------------
+    def sample(self, image=None, window=None, parameters=None):
+        if image is None:
+            image = self.make_model_image(window=window)
+        working_window = (image.window if window is None else window).copy()
+        if "window" in self.psf_mode:
+            raise NotImplementedError("PSF convolution in sub-window not available yet")
+        if self.psf_convolve_mode not in ("fft", "direct"):
+            raise ValueError(f"unrecognized psf_convolve_mode: {self.psf_convolve_mode}")
+        _engine.render(self, self._values(), _engine.RenderRequest(window=working_window, image=image))
+        if self.mask is not None:
+            image.data *= torch.logical_not(torch.as_tensor(self.mask, device=image.data.device))
+        return image
+
+    def jacobian(self, parameters: Optional[torch.Tensor] = None, as_representation: bool = False,
+                 window: Optional[Window] = None, pass_jacobian: Optional[Jacobian_Image] = None, **kw):
+        """_model_methods.py:260-396; writes straight into ``pass_jacobian``
+        (the group's Jacobian) when given."""
+        window = self.window if window is None else (self.window & window)
+        if parameters is not None:
+            if as_representation:
+                self.parameters.vector_set_representation(parameters)
+            else:
+                self.parameters.vector_set_values(parameters)
+        ids = list(self.parameters.vector_identities())
+        if pass_jacobian is None:
+            jac = self.target[window].jacobian_image(parameters=ids)
+        else:
+            jac = pass_jacobian
+            for pid in ids:
+                if pid not in jac.parameters:
+                    raise SpecificationConflict("pass_jacobian lacks this model's parameters")
+        if len(ids) == 0:
+            return jac
+        slots = self._free_slots()
+        slot_cols = {s: jac.parameters.index(pid) for (s, _), pid in zip(slots, ids)}
+        dvdr = {s: (d if as_representation else 1.0) for s, d in slots}
+        self._jacobian_windows(window, jac, slot_cols, dvdr)
+        return jac
+
+    def _jacobian_windows(self, window, jac, slot_cols, dvdr):
+        """_chunk_image_jacobian (_model_methods.py:350-396) splits windows wider
+        than image_chunksize; each piece renders on its own working grid."""
+        nx, ny = window.nx, window.ny
+        if max(nx, ny) > self.image_chunksize:
+            ncx = int(math.ceil(nx / self.image_chunksize))
+            ncy = int(math.ceil(ny / self.image_chunksize))
+            csx, csy = int(round(nx / ncx)), int(round(ny / ncy))
+            for ix in range(ncx):
+                for iy in range(ncy):
+                    sub = window.copy().crop_to_pixel(
+                        [[csx * ix, min(nx, csx * (ix + 1))], [csy * iy, min(ny, csy * (iy + 1))]])
+                    self._jacobian_windows(self.window & sub, jac, slot_cols, dvdr)
+            return
+        _engine.render(self, self._values(),
+                       _engine.RenderRequest(window=window, jac=jac, slot_cols=slot_cols, dvdr=dvdr))
+
+    def get_state(self):
+        state = {"name": self.name, "model_type": self.model_type, "window": self.window.get_state(),
+                 "parameters": self.parameters.get_state()}
+        for key in self.track_attrs:
+            if getattr(self, key) != getattr(self.__class__, key):
+                state[key] = getattr(self, key)
+        return state
+
+
+class Galaxy_Model(Component_Model):
+    """galaxy_model_object.py:20-116."""
+
+    model_type = f"galaxy {Component_Model.model_type}"
+    parameter_specs = {
+        "q": {"units": "b/a", "limits": (0, 1), "uncertainty": 0.03},
+        "PA": {"units": "radians", "limits": (0, np.pi), "cyclic": True, "uncertainty": 0.06},
+    }
+    _parameter_order = Component_Model._parameter_order + ("q", "PA")
+
+
+class Sersic_Galaxy(Galaxy_Model):
+    """sersic_model.py:41-91."""
+
+    model_type = f"sersic {Galaxy_Model.model_type}"
+    parameter_specs = {
+        "n": {"units": "none", "limits": (0.36, 8), "uncertainty": 0.05},
+        "Re": {"units": "arcsec", "limits": (0, None)},
+        "Ie": {"units": "log10(flux/arcsec^2)"},
+    }
+    _parameter_order = Galaxy_Model._parameter_order + ("n", "Re", "Ie")
+    _profile = "sersic"
+    usable = True
+
+    def total_flux(self, parameters=None, window=None):
+        p = self.parameters
+        return _engine.sersic_total_flux(10 ** float(p["Ie"].value), float(p["n"].value),
+                                         float(p["Re"].value), float(p["q"].value))
+
+
+class Exponential_Galaxy(Galaxy_Model):
+    """exponential_model.py:40-74."""
+
+    model_type = f"exponential {Galaxy_Model.model_type}"
+    parameter_specs = {"Ie": {"units": "log10(flux/arcsec^2)"}, "Re": {"units": "arcsec", "limits": (0, None)}}
+    _parameter_order = Galaxy_Model._parameter_order + ("Re", "Ie")
+    _profile = "exponential"
+    usable = True
+
+
+class Gaussian_Galaxy(Galaxy_Model):
+    """gaussian_model.py:37-70."""
+
+    model_type = f"gaussian {Galaxy_Model.model_type}"
+    parameter_specs = {"sigma": {"units": "arcsec", "limits": (0, None)}, "flux": {"units": "log10(flux)"}}
+    _parameter_order = Galaxy_Model._parameter_order + ("sigma", "flux")
+    _profile = "gaussian"
+    usable = True
+
+
+class Moffat_Galaxy(Galaxy_Model):
+    """moffat_model.py:23-68."""
+
+    model_type = f"moffat {Galaxy_Model.model_type}"
+    parameter_specs = {
+        "n": {"units": "none", "limits": (0.1, 10), "uncertainty": 0.05},
+        "Rd": {"units": "arcsec", "limits": (0, None)},
+        "I0": {"units": "log10(flux/arcsec^2)"},
+    }
+    _parameter_order = Galaxy_Model._parameter_order + ("n", "Rd", "I0")
+    _profile = "moffat"
+    usable = True
+
+
+class Sky_Model(Component_Model):
+    """sky_model_object.py:6-35: centre locked, no PSF, no integration."""
+
+    model_type = f"sky {Component_Model.model_type}"
+    parameter_specs = {"center": {"units": "arcsec", "locked": True, "uncertainty": 0.0}}
+
+    @property
+    def psf_mode(self):
+        return "none"
+
+    @psf_mode.setter
+    def psf_mode(self, val):
+        pass
+
+    @property
+    def integrate_mode(self):
+        return "none"
+
+    @integrate_mode.setter
+    def integrate_mode(self, val):
+        pass
+
+    def __init__(self, **kwargs):
+        super().__init__(**kwargs)
+        if self.parameters["center"].value is None:
+            self.parameters["center"]._value = self.window.center.clone()
+            self.parameters["center"].shape = (2,)
+
+
+class Flat_Sky(Sky_Model):
+    """flatsky_model.py:12-55."""
+
+    model_type = f"flat {Sky_Model.model_type}"
+    parameter_specs = {"F": {"units": "log10(flux/arcsec^2)"}}
+    _parameter_order = Sky_Model._parameter_order + ("F",)
+    _profile = "flat_sky"
+    usable = True
+
+
+class Group_Model(AstroPhot_Model):
+    """group_model_object.py:23-365."""
+
+    model_type = f"group {AstroPhot_Model.model_type}"
+    usable = True
+
+    def __init__(self, *, name=None, models: Optional[Sequence[AstroPhot_Model]] = None, **kwargs):
+        self.models = OrderedDict()
+        super().__init__(name=name, **kwargs)
+        if models is not None:
+            self.add_model(models)
+        self.update_window()
+
+    def add_model(self, model):
+        if isinstance(model, (tuple, list)):
+            for m in model:
+                self.add_model(m)
+            return
+        if model.name in self.models and model is not self.models[model.name]:
+            raise KeyError(f"{self.name} already has model with name {model.name}")
+        self.models[model.name] = model
+        self.parameters.link(model.parameters)
+        if self.target is None:
+            self._target = model.target
+        self.update_window()
+
+    def update_window(self, include_locked: bool = False):
+        """group_model_object.py:92-129 (single target)."""
+        w = None
+        for m in self.models.values():
+            if m.locked and not include_locked:
+                continue
+            w = m.window.copy() if w is None else (w | m.window)
+        self._window = w
+
+    @property
+    def psf_mode(self):
+        return "none"
+
+    def sample(self, image=None, window=None, parameters=None):
+        """group_model_object.py:183-231."""
+        if image is None:
+            sample_window = True
+            image = self.make_model_image(window=window)
+        else:
+            sample_window = False
+        if window is None:
+            window = image.window
+        working = image[window].blank_copy()
+        working.__class__ = Model_Image
+        for model in self.models.values():
+            if sample_window:
+                working += model(window=window)
+            else:
+                model(working, window=window)
+        image += working
+        return image
+
+    def jacobian(self, parameters: Optional[torch.Tensor] = None, as_representation: bool = False,
+                 pass_jacobian: Optional[Jacobian_Image] = None, window: Optional[Window] = None, **kw):
+        """group_model_object.py:233-283."""
+        if window is None:
+            window = self.window
+        if parameters is not None:
+            if as_representation:
+                self.parameters.vector_set_representation(parameters)
+            else:
+                self.parameters.vector_set_values(parameters)
+        jac = pass_jacobian
+        if jac is None:
+            jac = self.target[window].jacobian_image(parameters=list(self.parameters.vector_identities()))
+        for model in self.models.values():
+            model.jacobian(as_representation=as_representation, pass_jacobian=jac, window=window)
+        return jac
+
+    def fit_mask(self):
+        return torch.zeros_like(self.target[self.window].mask)
+
+    def __iter__(self):
+        return iter(self.models.values())

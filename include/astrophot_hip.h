@@ -1,0 +1,183 @@
+/*
+ * astrophot_hip.h -- C ABI of the MI355X (gfx950) image-model hot path.
+ *
+ * Plain pointers and sizes only.  Every pointer argument is device memory
+ * owned by the caller unless stated otherwise (structs are passed by host
+ * pointer and copied at launch); every call is asynchronous on the HIP stream
+ * passed as ``stream`` (a hipStream_t; NULL = default stream) and returns 0 on
+ * success or a negative APX_E* code, with a message in apx_last_error().
+ * No call allocates device memory: scratch space is passed in by the caller
+ * and sized with the matching *_workspace_bytes() query.
+ *
+ * Reference interfaces replaced (paths relative to the AstroPhot repository):
+ *   apx_render_component  -> models/_model_methods.py:82-184 (_sample_init,
+ *                            _sample_integrate) with models/_shared_methods.py:
+ *                            274-418 and utils/operations.py:94-247
+ *   apx_psf_shift         -> models/_model_methods.py:187-207, 243
+ *                            (_shift_psf + normalise), utils/interpolate.py:282-329
+ *   apx_convolve          -> models/_model_methods.py:233-257 (_sample_convolve),
+ *                            utils/operations.py:9-36, models/model_object.py:
+ *                            342-373 (crop, reduce(1), image +=)
+ *   apx_chi2              -> fit/lm.py:373-378 (_chi2), utils/optimization.py:6-30
+ *   apx_normal_equations  -> fit/lm.py:387-399 (_hess, _grad)
+ *   apx_rpp               -> fit/lm.py:401-406 (_rpp)
+ *   apx_lm_solve          -> fit/lm.py:359-371 (_h, damped solve)
+ */
+#ifndef ASTROPHOT_HIP_H
+#define ASTROPHOT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APX_OK 0
+#define APX_E_INVALID (-1) /* bad argument (shape, mode, pointer) */
+#define APX_E_HIP (-2)     /* a HIP runtime call failed */
+#define APX_E_LIMIT (-3)   /* a size exceeds a documented limit */
+
+/* profile kinds: models/sersic_model.py, exponential_model.py,
+ * gaussian_model.py, moffat_model.py, flatsky_model.py */
+#define APX_SERSIC 0
+#define APX_EXPONENTIAL 1
+#define APX_GAUSSIAN 2
+#define APX_MOFFAT 3
+#define APX_FLAT_SKY 4
+
+/* natural parameter slots of a component */
+#define APX_P_CX 0
+#define APX_P_CY 1
+#define APX_P_Q 2
+#define APX_P_PA 3
+#define APX_P_A 4 /* sersic n | exponential Re | gaussian sigma | moffat n | sky F */
+#define APX_P_B 5 /* sersic Re | exponential Ie | gaussian flux | moffat Rd */
+#define APX_P_C 6 /* sersic Ie | moffat I0 */
+#define APX_NSLOT 7
+
+#define APX_SAMPLE_MIDPOINT 0
+#define APX_SAMPLE_QUAD 1
+
+#define APX_MAX_QUAD_LEVEL 7 /* quad_level**2 <= 64 lanes */
+#define APX_MAX_GRIDDING 8   /* gridding**2 <= 64 lanes */
+#define APX_MAX_DEPTH 3
+#define APX_MAX_KERNEL 131   /* shifted-PSF side (PSF side + 2) */
+#define APX_MAX_CHAN 8       /* render planes: value + up to 7 slots */
+
+/* One component rendered on its working grid.  Geometry is float64 from the
+ * host.  Working-grid pixel (i, j) (i = column, j = row) sits at plane offset
+ * (X, Y) = P . (i - ci, j - cj) + frac from the model centre. */
+typedef struct apx_component {
+  int32_t profile;        /* APX_SERSIC .. APX_FLAT_SKY */
+  int32_t sampling;       /* APX_SAMPLE_MIDPOINT | APX_SAMPLE_QUAD */
+  int32_t sample_quad;    /* N of sampling_mode "quad:N" */
+  int32_t integrate;      /* 0 = integrate_mode "none", 1 = "threshold" */
+  int32_t quad_level;     /* integrate_quad_level */
+  int32_t gridding;       /* integrate_gridding */
+  int32_t max_depth;      /* integrate_max_depth */
+  int32_t mean_reference; /* 1: reference = mean of the initial samples
+                             (_model_methods.py:151-152); 0: use `reference` */
+  int32_t nx, ny;         /* working grid */
+  int32_t ci, cj;         /* integer centre pixel on the working grid */
+  int32_t nchan;          /* render planes: 1 (value) + Jacobian channels */
+  int32_t chan_slot[APX_MAX_CHAN]; /* slot of render plane c (c >= 1) */
+  double P[4];            /* pixelscale matrix, row-major */
+  double frac[2];
+  double pixel_area;
+  double softening;
+  double tolerance;       /* sampling_tolerance */
+  double reference;       /* integrate reference when mean_reference == 0 */
+  double q, pa;           /* axis ratio, position angle (radians) */
+  double prm[3];          /* slot A, B, C values */
+  double dvdr[APX_NSLOT]; /* d value / d representation per slot */
+  /* Gauss-Legendre abscissae/weights on [-1, 1] (utils/operations.py:105-120)
+   * for integrate_quad_level (gl_*) and for sampling "quad:N" (sq_*) */
+  double gl_node[8], gl_weight[8];
+  double sq_node[8], sq_weight[8];
+} apx_component;
+
+/* Workspace (bytes) needed by apx_render_component for npix working pixels. */
+int64_t apx_render_workspace_bytes(int64_t npix);
+
+/* Render one component on its working grid into `planes` (nchan x ny x nx
+ * float32).  Plane 0 is the value; plane c >= 1 is d value / d rep of slot
+ * chan_slot[c].  Midpoint or quad:N sampling, then threshold-selected adaptive
+ * Gauss-Legendre integration; selection decisions and supersampled pixels are
+ * evaluated in float64. */
+int apx_render_component(const apx_component* comp, float* planes, void* workspace,
+                         int64_t workspace_bytes, void* stream);
+
+/* Number of pixels the last apx_render_component on `workspace` integrated
+ * (copies one int to the host; synchronises the stream). */
+int apx_render_selected_count(const void* workspace, void* stream, int32_t* count);
+
+/* Bilinear sub-pixel shift of a (ph x pw) float32 PSF with a 1-px zero pad,
+ * normalised to unit sum, flipped for direct correlation, and its derivatives
+ * with respect to the model centre in representation units:
+ *   dK/dc_x = (dK/ds_x pinv[0] + dK/ds_y pinv[2]) dvdr_x ,
+ *   dK/dc_y = (dK/ds_x pinv[1] + dK/ds_y pinv[3]) dvdr_y .
+ * Output `kernels`: 3 x (ph+2) x (pw+2) float32 [K, dK/dcx, dK/dcy]. */
+int apx_psf_shift(const float* psf, int32_t ph, int32_t pw, double shift_x, double shift_y,
+                  const double* pinv /* host, 4 doubles */, double dvdr_x, double dvdr_y, float* kernels,
+                  void* stream);
+
+/* Direct convolution of render planes, cropped to an output box. */
+typedef struct apx_conv_job {
+  int32_t nx, ny, nchan;   /* render planes (working grid) */
+  int32_t kh, kw;          /* kernel side (odd); 1 x 1 = plain crop/copy */
+  int32_t x0, y0;          /* working-grid pixel under output pixel (0, 0) */
+  int32_t ow, oh;          /* output box */
+  float* img;              /* value destination (+=), may be NULL */
+  int64_t img_row;         /* img row stride (floats), >= img_w */
+  int32_t img_w, img_h;    /* img extent (pixels); the box must fit inside */
+  int32_t img_x0, img_y0;  /* img pixel of output (0, 0) */
+  float* jac;              /* Jacobian destination (+=), pixel-major, may be NULL */
+  int64_t jac_row;         /* jac row stride (floats), >= jac_w * jac_np */
+  int32_t jac_w, jac_h;    /* jac extent (pixels) */
+  int32_t jac_np;          /* jac pixel stride (floats) */
+  int32_t jac_x0, jac_y0;  /* jac pixel of output (0, 0) */
+  int32_t chan_col[APX_MAX_CHAN]; /* jac column of render plane c (c >= 1) */
+  int32_t dk_col[2];       /* jac columns of conv(plane 0, dK/dc), -1 = none */
+} apx_conv_job;
+
+/* out(ox, oy) = sum_ab kernel[a][b] * plane[y0+oy-(kh-1)/2+a][x0+ox-(kw-1)/2+b]
+ * (kernels pre-flipped by apx_psf_shift).  `kernels` is 3 x kh x kw when
+ * dk_col is used, else 1 x kh x kw. */
+int apx_convolve(const apx_conv_job* job, const float* planes, const float* kernels,
+                 void* stream);
+
+/* Workspace (bytes) for apx_chi2 / apx_normal_equations / apx_rpp. */
+int64_t apx_reduce_workspace_bytes(int64_t npix, int32_t np);
+
+/* out[0] = sum over unmasked pixels of weight*(data-model)^2 (float64, device).
+ * mask (uint8, nonzero = ignore) and weight may be NULL. */
+int apx_chi2(const float* model, const float* data, const float* weight, const uint8_t* mask,
+             int64_t npix, double* out, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* hess = J^T W J (np x np) and grad = -J^T W (model - data) (np), float64,
+ * with J pixel-major (npix x np float32).  FP32 matrix-core products,
+ * float64 sum over workgroups.  np <= 16. */
+int apx_normal_equations(const float* J, const float* model, const float* data, const float* weight,
+                         const uint8_t* mask, int64_t npix, int32_t np, double* hess, double* grad,
+                         void* workspace, int64_t workspace_bytes, void* stream);
+
+/* rpp = J^T ((2/d) (W (m1 - m0) / d - W (J h)))  (fit/lm.py:281, 401-406);
+ * h and rpp float64 (np). */
+int apx_rpp(const float* J, const float* m0, const float* m1, const float* weight,
+            const uint8_t* mask, int64_t npix, int32_t np, const double* h, double d, double* rpp,
+            void* workspace, int64_t workspace_bytes, void* stream);
+
+/* h = A(L)^-1 g with A_ii = H_ii + L (1 + H_ii), A_ij = H_ij / (1 + L): float64
+ * Cholesky on one workgroup (np <= 64).  status[0] = 0 ok, k+1 if the pivot of
+ * row k is not positive (h is then NaN). */
+int apx_lm_solve(const double* hess, const double* grad, int32_t np, double L, double* h,
+                 int32_t* status, void* stream);
+
+/* Last error message of the calling thread. */
+const char* apx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTROPHOT_HIP_H */

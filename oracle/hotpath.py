@@ -1,0 +1,862 @@
+"""Float64 CPU restatement of AstroPhot's image-model hot path (test infrastructure).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg may import this module.  It is the checker for the HIP path and never part
+of it.
+
+It restates, in torch float64 on the CPU, what AstroPhot computes for
+``model.sample`` / ``model.jacobian`` of Sersic, exponential, Gaussian and
+Moffat galaxies plus a flat sky, with PSF convolution, and for ``fit.LM``.
+The Jacobian is taken the way the reference takes it: forward-mode automatic
+differentiation through the whole render (``_model_methods.py:325-340``), one
+dual-number pass per free parameter.
+
+Pinning (what checks each function; see ``tests/test_oracle.py``):
+
+=========================  ==================================================
+function                   pinned by
+=========================  ==================================================
+sersic_n_to_b, flux        reference KATs ``test_utils.py:344-434``; brute-force
+                           integral of the profile on a fine grid
+profiles                   closed forms (Gaussian total flux, Moffat flux,
+                           exponential = Sersic n=1)
+boundaries / inverse       reference KATs ``test_parameter.py:117-161``
+                           (round trip, cyclic wrap); closed-form derivatives
+quad_table                 scipy ``roots_legendre`` (the reference's own call,
+                           ``operations.py:110``); exact on polynomials deg<=5
+grid_integrate             golden vectors written by running the reference's
+                           ``operations.py`` (``tests/golden/``); erf closed form
+                           of a Gaussian pixel integral
+curvature                  brute force 5-point stencil, replicate border
+interp2d / shift_psf       brute-force bilinear formula incl. edge extrapolation
+fft convolution            scipy ``fftconvolve(mode="same")`` (reference KAT
+                           ``test_utils.py:15-38``) and direct conv2d
+component_sample           flux conservation, whole-pixel translation, q=1
+                           PA-independence, symmetry invariants
+jacobian (forward AD)      central finite differences, reverse-mode autograd
+chi2                       reference KATs ``test_utils.py:55-121`` (4.5, 0.75,
+                           5, 5/7, 9, 1.5, 10, 10/7)
+LM damping / solve         numpy.linalg.solve of the damped matrix; fit recovers
+                           truth on noiseless data
+LM accept/reject sequence  parity unpinned beyond the above: restated line by
+                           line from ``fit/lm.py:248-493``
+=========================  ==================================================
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+import torch.autograd.forward_ad as fwAD
+import torch.nn.functional as F
+from scipy.special import roots_legendre
+
+F64 = torch.float64
+NORTH = math.pi / 2  # image/image_header.py:36
+
+
+# ---------------------------------------------------------------------------
+# parameter maps: utils/conversions/optimization.py:6-54
+# ---------------------------------------------------------------------------
+def boundaries(v, lo, hi):
+    """value -> representation (optimization.py:6-14)."""
+    v = torch.as_tensor(v, dtype=F64)
+    if lo is None:
+        return v - 1.0 / (v - hi)
+    if hi is None:
+        return v - 1.0 / (v - lo)
+    return torch.tan((v - lo) * math.pi / (hi - lo) - math.pi / 2)
+
+
+def inv_boundaries(r, lo, hi):
+    """representation -> value (optimization.py:17-25)."""
+    r = torch.as_tensor(r, dtype=F64)
+    if lo is None:
+        return (r + hi - torch.sqrt(torch.pow(r - hi, 2) + 4)) * 0.5
+    if hi is None:
+        return (r + lo + torch.sqrt(torch.pow(r - lo, 2) + 4)) * 0.5
+    return (torch.arctan(r) + math.pi / 2) * (hi - lo) / math.pi + lo
+
+
+def cyclic_boundaries(v, lo, hi):
+    """optimization.py:50-54 (python-style modulo, like torch ``%``)."""
+    v = torch.as_tensor(v, dtype=F64)
+    return lo + torch.remainder(v - lo, hi - lo)
+
+
+# ---------------------------------------------------------------------------
+# parametric profiles: utils/parametric_profiles.py, utils/conversions/functions.py
+# ---------------------------------------------------------------------------
+def sersic_n_to_b(n):
+    """conversions/functions.py:7-22."""
+    return (
+        2 * n
+        - 1 / 3
+        + 4 / (405 * n)
+        + 46 / (25515 * n**2)
+        + 131 / (1148175 * n**3)
+        - 2194697 / (30690717750 * n**4)
+    )
+
+
+def sersic_Ie_to_flux(Ie, n, R, q):
+    """conversions/functions.py:168-190 (Ie in flux/arcsec^2, not log)."""
+    bn = sersic_n_to_b(n)
+    return 2 * math.pi * Ie * R**2 * q * n * (torch.exp(bn) * bn ** (-2 * n)) * torch.exp(
+        torch.special.gammaln(2 * n)
+    )
+
+
+def sersic(R, n, Re, Ie):
+    """parametric_profiles.py:7-18."""
+    bn = sersic_n_to_b(n)
+    return Ie * torch.exp(-bn * (torch.pow(R / Re, 1 / n) - 1))
+
+
+def exponential(R, Re, Ie):
+    """parametric_profiles.py:63-74 (b(1) from the Sersic polynomial)."""
+    return Ie * torch.exp(-sersic_n_to_b(torch.tensor(1.0, dtype=F64)) * ((R / Re) - 1.0))
+
+
+def gaussian(R, sigma, I0):
+    """parametric_profiles.py:39-48 (1-D normalisation quirk kept)."""
+    return (I0 / torch.sqrt(2 * np.pi * sigma**2)) * torch.exp(-0.5 * torch.pow(R / sigma, 2))
+
+
+def moffat(R, n, Rd, I0):
+    """parametric_profiles.py:89-100."""
+    return I0 / (1 + (R / Rd) ** 2) ** n
+
+
+# model kind -> ordered leaf parameters (name, size, lo, hi, cyclic)
+# galaxy_model_object.py:44-53; sersic_model.py:61-66; exponential_model.py:55-59;
+# gaussian_model.py:53-57; moffat_model.py:43-48; flatsky_model.py / sky_model_object.py
+_GAL = [("center", 2, None, None, False), ("q", 1, 0.0, 1.0, False), ("PA", 1, 0.0, math.pi, True)]
+SPECS = {
+    "sersic": _GAL + [("n", 1, 0.36, 8.0, False), ("Re", 1, 0.0, None, False), ("Ie", 1, None, None, False)],
+    "exponential": _GAL + [("Re", 1, 0.0, None, False), ("Ie", 1, None, None, False)],
+    "gaussian": _GAL + [("sigma", 1, 0.0, None, False), ("flux", 1, None, None, False)],
+    "moffat": _GAL + [("n", 1, 0.1, 10.0, False), ("Rd", 1, 0.0, None, False), ("I0", 1, None, None, False)],
+    "flat_sky": [("center", 2, None, None, False), ("F", 1, None, None, False)],
+}
+
+
+def radial(kind, R, v, area):
+    """_shared_methods.py:312-408: profile times pixel_area * 10**log-brightness."""
+    if kind == "sersic":
+        return sersic(R, v["n"], v["Re"], area * 10 ** v["Ie"])
+    if kind == "exponential":
+        return exponential(R, v["Re"], area * 10 ** v["Ie"])
+    if kind == "gaussian":
+        return gaussian(R, v["sigma"], area * 10 ** v["flux"])
+    if kind == "moffat":
+        return moffat(R, v["n"], v["Rd"], area * 10 ** v["I0"])
+    raise ValueError(kind)
+
+
+# ---------------------------------------------------------------------------
+# pixel <-> plane geometry: image/wcs.py:432-702 (PPCS), image/window_object.py
+# ---------------------------------------------------------------------------
+class Grid:
+    """A window: pixel_shape (nx, ny), pixelscale matrix P, reference pixel
+    ``ref_ij`` pinned to plane point ``ref_xy`` (wcs.py:561-584)."""
+
+    def __init__(self, nx, ny, P=1.0, ref_ij=(-0.5, -0.5), ref_xy=(0.0, 0.0)):
+        self.nx, self.ny = int(nx), int(ny)
+        P = torch.as_tensor(P, dtype=F64)
+        if P.numel() == 1:
+            P = torch.eye(2, dtype=F64) * P.reshape(())
+        self.P = P.clone()
+        self.ref_ij = torch.as_tensor(ref_ij, dtype=F64).clone()
+        self.ref_xy = torch.as_tensor(ref_xy, dtype=F64).clone()
+
+    @property
+    def Pinv(self):
+        return torch.linalg.inv(self.P)
+
+    @property
+    def area(self):
+        return torch.linalg.det(self.P).abs()
+
+    def copy(self):
+        return Grid(self.nx, self.ny, self.P, self.ref_ij.clone(), self.ref_xy.clone())
+
+    def pixel_to_plane(self, ij):
+        return self.P @ (ij - self.ref_ij) + self.ref_xy
+
+    def plane_to_pixel(self, xy):
+        return self.Pinv @ (xy - self.ref_xy) + self.ref_ij
+
+    def meshgrid(self):
+        """window_object.py:390-405: plane coords of pixel centres, (2, ny, nx)."""
+        i = torch.arange(self.nx, dtype=F64)
+        j = torch.arange(self.ny, dtype=F64)
+        mi, mj = torch.meshgrid(i, j, indexing="xy")
+        c = self.P @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.ref_ij.view(2, 1))
+        c = c + self.ref_xy.view(2, 1)
+        return c.reshape(2, self.ny, self.nx)
+
+    @property
+    def origin(self):
+        return self.pixel_to_plane(-0.5 * torch.ones(2, dtype=F64))
+
+    @property
+    def end(self):
+        return self.P @ torch.tensor([self.nx, self.ny], dtype=F64)
+
+    def pad(self, bx, by):
+        """window_object.py:367-388 (two-value form)."""
+        self.nx += 2 * bx
+        self.ny += 2 * by
+        self.ref_ij = self.ref_ij + torch.tensor([bx, by], dtype=F64)
+        return self
+
+    def crop(self, bx, by):
+        """window_object.py:335-356 (two-value form)."""
+        self.nx -= 2 * bx
+        self.ny -= 2 * by
+        self.ref_ij = self.ref_ij - torch.tensor([bx, by], dtype=F64)
+        return self
+
+    def crop_to_pixel(self, pixels):
+        """window_object.py:358-365; pixels = [[xmin,xmax],[ymin,ymax]]."""
+        p = torch.tensor(pixels, dtype=F64)
+        self.ref_ij = self.ref_ij - p[:, 0]
+        self.nx, self.ny = int(round(float(p[0, 1] - p[0, 0]))), int(round(float(p[1, 1] - p[1, 0])))
+        return self
+
+    def _from_pixel_box(self, lo_pix, end_pix):
+        new = self.copy()
+        shape = end_pix - lo_pix
+        new.nx, new.ny = int(torch.round(shape[0])), int(torch.round(shape[1]))
+        # copy(origin=...) re-pins ref_ij=(-0.5,-0.5) at the new origin (window_object.py:143-150)
+        new.ref_xy = self.pixel_to_plane(lo_pix)
+        new.ref_ij = torch.tensor([-0.5, -0.5], dtype=F64)
+        return new
+
+    def __and__(self, other):
+        """window_object.py:511-523."""
+        o = self.plane_to_pixel(other.origin)
+        lo = torch.maximum(-0.5 * torch.ones_like(o), o)
+        e = self.plane_to_pixel(other.origin + other.end)
+        hi = torch.minimum(torch.tensor([self.nx, self.ny], dtype=F64) - 0.5, e)
+        return self._from_pixel_box(lo, hi)
+
+    def __or__(self, other):
+        """window_object.py:483-495."""
+        o = self.plane_to_pixel(other.origin)
+        lo = torch.minimum(-0.5 * torch.ones_like(o), o)
+        e = self.plane_to_pixel(other.origin + other.end)
+        hi = torch.maximum(torch.tensor([self.nx, self.ny], dtype=F64), e)
+        return self._from_pixel_box(lo, hi)
+
+    @staticmethod
+    def indices(ref, obj):
+        """window_object.py:241-253: slices into ``ref``'s data covering ``obj``."""
+        o = torch.round(ref.plane_to_pixel(obj.origin) + 0.5).int()
+        o = torch.maximum(torch.zeros_like(o), o)
+        e = torch.round(ref.plane_to_pixel(obj.origin + obj.end) + 0.5).int()
+        e = torch.minimum(torch.tensor([ref.nx, ref.ny], dtype=torch.int32), e)
+        return slice(int(o[1]), int(e[1])), slice(int(o[0]), int(e[0]))
+
+
+def add_image(dst, dst_grid, src, src_grid):
+    """image_object.py:450-457 window-aware ``+=``."""
+    di = Grid.indices(dst_grid, src_grid)
+    si = Grid.indices(src_grid, dst_grid)
+    dst = dst.clone()
+    dst[di] = dst[di] + src[si]
+    return dst
+
+
+# ---------------------------------------------------------------------------
+# quadrature: utils/operations.py:94-247
+# ---------------------------------------------------------------------------
+def displacement_grid(N, P):
+    """operations.py:94-102."""
+    p = torch.linspace(-(N - 1) / (2 * N), (N - 1) / (2 * N), N, dtype=F64)
+    PX, PY = torch.meshgrid(p, p, indexing="xy")
+    return (P @ torch.stack((PX, PY)).view(2, -1)).reshape((2, *PX.shape))
+
+
+def quad_table(n, P):
+    """operations.py:105-120."""
+    a, w = roots_legendre(n)
+    w = torch.tensor(w, dtype=F64)
+    a = torch.tensor(a, dtype=F64)
+    X, Y = torch.meshgrid(a, a, indexing="xy")
+    W = torch.outer(w, w) / 4.0
+    XY = P @ (torch.stack((X, Y)).view(2, -1) / 2.0)
+    return XY[0], XY[1], W.reshape(-1)
+
+
+def single_quad_integrate(X, Y, P, evalf, n=3):
+    """operations.py:123-147; ``evalf(X, Y, P)`` includes the pixel area of P."""
+    ax, ay, w = quad_table(n, P)
+    Xs = X[..., None] + ax
+    Ys = Y[..., None] + ay
+    res = evalf(Xs, Ys, P)
+    ref = res[..., (n**2) // 2]
+    return (res * w).sum(dim=-1), ref
+
+
+def grid_integrate(X, Y, P, evalf, quad_level=3, gridding=5, depth=1, max_depth=2, reference=None):
+    """operations.py:150-247 (threshold multiplied by gridding**2 per level, :241)."""
+    res, ref = single_quad_integrate(X, Y, P, evalf, quad_level)
+    if depth >= max_depth:
+        return res
+    integral = torch.zeros_like(X)
+    select = torch.abs(res - ref) > reference
+    keep = torch.logical_not(select)
+    integral[keep] = res[keep]
+    step = displacement_grid(gridding, P)
+    sx = X[select].unsqueeze(-1) + step[0].reshape(-1)
+    sy = Y[select].unsqueeze(-1) + step[1].reshape(-1)
+    sub = grid_integrate(sx, sy, P / gridding, evalf, quad_level, gridding, depth + 1, max_depth,
+                         reference * gridding**2)
+    integral[select] = sub.sum(dim=-1)
+    return integral
+
+
+# ---------------------------------------------------------------------------
+# PSF shift and convolution: _model_methods.py:187-257, interpolate.py:282-343,
+# operations.py:9-36
+# ---------------------------------------------------------------------------
+CURVATURE = torch.tensor([[0.0, 1.0, 0.0], [1.0, -4.0, 1.0], [0.0, 1.0, 0.0]], dtype=F64)
+
+
+def curvature(mid):
+    """_model_methods.py:87-97: valid Laplacian, replicate-padded back."""
+    c = F.conv2d(mid.view(1, 1, *mid.shape), CURVATURE.view(1, 1, 3, 3), padding="valid")
+    return F.pad(c, (1, 1, 1, 1), mode="replicate").squeeze(0).squeeze(0)
+
+
+def interp2d(im, x, y):
+    """interpolate.py:282-329: bilinear with clamped corners (extrapolates)."""
+    h, w = im.shape
+    shape = x.shape
+    x = x.reshape(-1)
+    y = y.reshape(-1)
+    x0 = x.floor().long()
+    y0 = y.floor().long()
+    x1 = x0 + 1
+    y1 = y0 + 1
+    x0 = x0.clamp(0, w - 2)
+    x1 = x1.clamp(1, w - 1)
+    y0 = y0.clamp(0, h - 2)
+    y1 = y1.clamp(1, h - 1)
+    fa, fb, fc, fd = im[y0, x0], im[y1, x0], im[y0, x1], im[y1, x1]
+    wa = (x1 - x) * (y1 - y)
+    wb = (x1 - x) * (y - y0)
+    wc = (x - x0) * (y1 - y)
+    wd = (x - x0) * (y - y0)
+    return (fa * wa + fb * wb + fc * wc + fd * wd).view(*shape)
+
+
+def shift_psf(psf, shift):
+    """_model_methods.py:187-207 (bilinear, keep_pad=True)."""
+    p = F.pad(psf, (1, 1, 1, 1))
+    X, Y = torch.meshgrid(
+        torch.arange(p.shape[1], dtype=F64) - shift[0],
+        torch.arange(p.shape[0], dtype=F64) - shift[1],
+        indexing="xy",
+    )
+    return interp2d(p, X.clone(), Y.clone())
+
+
+def fft_convolve(img, psf):
+    """operations.py:9-36 with img_prepadded=True."""
+    s = img.shape
+    conv = torch.fft.irfft2(torch.fft.rfft2(img, s=s) * torch.fft.rfft2(psf, s=s), s=s)
+    return torch.roll(conv, shifts=(-int((psf.shape[0] - 1) / 2), -int((psf.shape[1] - 1) / 2)),
+                      dims=(0, 1))[: s[0], : s[1]]
+
+
+def direct_convolve(img, psf):
+    """_model_methods.py:247-255."""
+    return F.conv2d(img.view(1, 1, *img.shape), torch.flip(psf.view(1, 1, *psf.shape), dims=(2, 3)),
+                    padding="same").squeeze(0).squeeze(0)
+
+
+def psf_border(psf):
+    """psf_image.py:71-93: ceil((1 + shape)/2) in (x, y) order."""
+    return int(math.ceil((1 + psf.shape[1]) / 2)), int(math.ceil((1 + psf.shape[0]) / 2))
+
+
+# ---------------------------------------------------------------------------
+# components
+# ---------------------------------------------------------------------------
+@dataclass
+class Component:
+    """A Component_Model's knobs (model_object.py:63-111) and parameters."""
+
+    kind: str
+    window: Grid
+    values: Dict[str, torch.Tensor]
+    locked: Sequence[str] = ()
+    psf: Optional[torch.Tensor] = None
+    psf_mode: str = "none"
+    psf_convolve_mode: str = "fft"
+    sampling_mode: str = "midpoint"
+    sampling_tolerance: float = 1e-2
+    integrate_mode: str = "threshold"
+    integrate_max_depth: int = 3
+    integrate_gridding: int = 5
+    integrate_quad_level: int = 3
+    softening: float = 1e-3
+    image_chunksize: int = 1000
+
+    def __post_init__(self):
+        self.values = {k: torch.as_tensor(v, dtype=F64).clone() for k, v in self.values.items()}
+        if self.kind == "flat_sky":
+            # sky_model_object.py:12-35: centre locked, no psf, no integration
+            self.locked = tuple(set(self.locked) | {"center"})
+            self.psf_mode = "none"
+            self.integrate_mode = "none"
+            if "center" not in self.values:
+                self.values["center"] = self.window.pixel_to_plane(
+                    torch.tensor([(self.window.nx - 1) / 2, (self.window.ny - 1) / 2], dtype=F64))
+
+    @property
+    def free(self):
+        return [s for s in SPECS[self.kind] if s[0] not in self.locked]
+
+    def vector_values(self):
+        return torch.cat([self.values[s[0]].reshape(-1) for s in self.free])
+
+    def val_to_rep(self, vals):
+        out, k = [], 0
+        for name, size, lo, hi, cyc in self.free:
+            v = vals[k:k + size]
+            k += size
+            if cyc:
+                out.append(cyclic_boundaries(v, lo, hi))
+            elif lo is None and hi is None:
+                out.append(v)
+            else:
+                out.append(boundaries(v, lo, hi))
+        return torch.cat(out)
+
+    def rep_to_values(self, rep):
+        """parameter.py:330-374; returns a name -> value dict (locked ones fixed)."""
+        vals = dict(self.values)
+        k = 0
+        for name, size, lo, hi, cyc in self.free:
+            r = rep[k:k + size]
+            k += size
+            if cyc:
+                v = cyclic_boundaries(r, lo, hi)
+            elif lo is None and hi is None:
+                v = r
+            else:
+                v = inv_boundaries(r, lo, hi)
+            vals[name] = v if size > 1 else v.reshape(())
+        return vals
+
+    @property
+    def nparams(self):
+        return sum(s[1] for s in self.free)
+
+    # -- evaluate_model ---------------------------------------------------
+    def evaluate(self, X, Y, P, v):
+        """_shared_methods.py:286-307 + _model_methods.py:38-40; flatsky_model.py:53-55."""
+        area = torch.linalg.det(P).abs()
+        if self.kind == "flat_sky":
+            return torch.ones_like(X) * (area * 10 ** v["F"])
+        theta = -(v["PA"] - NORTH)
+        s, c = torch.sin(theta), torch.cos(theta)
+        Xr, Yr = c * X - s * Y, s * X + c * Y
+        Yr = Yr / v["q"]
+        R = torch.sqrt(Xr**2 + Yr**2 + self.softening**2)
+        return radial(self.kind, R, v, area)
+
+    def integrate_reference(self, deep, v):
+        """_model_methods.py:151-152; Sersic override sersic_model.py:78-89."""
+        if self.kind == "sersic":
+            return sersic_Ie_to_flux(10 ** v["Ie"], v["n"], v["Re"], v["q"]) / deep.numel()
+        return torch.sum(deep) / deep.numel()
+
+
+def sample_init(comp, grid, v, center):
+    """_model_methods.py:82-148 (midpoint and quad:N)."""
+    Coords = grid.meshgrid()
+    X, Y = Coords[0] - center[0], Coords[1] - center[1]
+    ev = lambda x, y, P: comp.evaluate(x, y, P, v)
+    if comp.sampling_mode == "midpoint":
+        mid = ev(X, Y, grid.P)
+        return mid + curvature(mid), mid
+    if comp.sampling_mode.startswith("quad"):
+        n = int(comp.sampling_mode.split(":")[1])
+        res, ref = single_quad_integrate(X, Y, grid.P, ev, n)
+        return ref, res
+    raise ValueError(f"unknown sampling mode {comp.sampling_mode}")
+
+
+def sample_integrate(comp, deep, reference, grid, v, center):
+    """_model_methods.py:155-184."""
+    if comp.integrate_mode == "none":
+        return deep
+    Coords = grid.meshgrid()
+    X, Y = Coords[0] - center[0], Coords[1] - center[1]
+    ref = comp.integrate_reference(deep, v)
+    select = torch.abs(deep - reference) > (comp.sampling_tolerance * ref)
+    ev = lambda x, y, P: comp.evaluate(x, y, P, v)
+    intdeep = grid_integrate(X[select], Y[select], grid.P, ev, comp.integrate_quad_level,
+                             comp.integrate_gridding, 1, comp.integrate_max_depth,
+                             comp.sampling_tolerance * ref)
+    deep = deep.clone()
+    deep[select] = intdeep
+    return deep
+
+
+def component_sample(comp, v, sample_window=None):
+    """Component_Model.sample with image=None (model_object.py:258-375).
+
+    Returns (data, grid) over ``comp.window & sample_window`` (or comp.window).
+    """
+    if sample_window is None:
+        img_grid = comp.window.copy()
+        working = comp.window.copy()
+    else:
+        img_grid = comp.window & sample_window   # core_model.py:193-204
+        working = sample_window.copy()
+    center = v["center"]
+    if comp.psf_mode == "full":
+        psf = comp.psf
+        bx, by = psf_border(psf)
+        working.pad(bx, by)
+        pc = working.plane_to_pixel(center)
+        shift = pc - torch.round(pc)
+        working.ref_ij = working.ref_ij - shift          # window_object.py:284-290
+        reference, deep = sample_init(comp, working, v, center)
+        deep = sample_integrate(comp, deep, reference, working, v, center)
+        K = shift_psf(psf, shift)
+        K = K / torch.sum(K)
+        if comp.psf_convolve_mode == "fft":
+            data = fft_convolve(deep, K)
+        else:
+            data = direct_convolve(deep, K)
+        working.ref_ij = working.ref_ij + shift
+        data = data[by:data.shape[0] - by, bx:data.shape[1] - bx]
+        working.crop(bx, by)
+    else:
+        reference, deep = sample_init(comp, working, v, center)
+        data = sample_integrate(comp, deep, reference, working, v, center)
+    out = torch.zeros((img_grid.ny, img_grid.nx), dtype=F64)
+    return add_image(out, img_grid, data, working), img_grid
+
+
+# ---------------------------------------------------------------------------
+# models: a single component or a group (group_model_object.py:183-283)
+# ---------------------------------------------------------------------------
+class Model:
+    """Sum of components over ``window`` with a flat parameter vector in
+    insertion order (param/base.py:181-201)."""
+
+    def __init__(self, components: List[Component], window: Optional[Grid] = None, group: bool = None):
+        self.components = list(components)
+        self.group = len(self.components) > 1 if group is None else group
+        if window is None:
+            window = self.components[0].window.copy()
+            for c in self.components[1:]:
+                window = window | c.window          # group_model_object.py:92-129
+        self.window = window
+        self.sizes = [c.nparams for c in self.components]
+
+    @property
+    def nparams(self):
+        return sum(self.sizes)
+
+    def vector_values(self):
+        return torch.cat([c.vector_values() for c in self.components])
+
+    def vector_representation(self):
+        return torch.cat([c.val_to_rep(c.vector_values()) for c in self.components])
+
+    def split(self, rep):
+        out, k = [], 0
+        for n in self.sizes:
+            out.append(rep[k:k + n])
+            k += n
+        return out
+
+    def forward(self, rep):
+        """model(parameters=rep, as_representation=True).data."""
+        if not self.group:
+            c = self.components[0]
+            data, g = component_sample(c, c.rep_to_values(rep))
+            out = torch.zeros((self.window.ny, self.window.nx), dtype=F64)
+            return add_image(out, self.window, data, g)
+        out = torch.zeros((self.window.ny, self.window.nx), dtype=F64)
+        for c, r in zip(self.components, self.split(rep)):
+            data, g = component_sample(c, c.rep_to_values(r), sample_window=self.window)
+            out = add_image(out, self.window, data, g)
+        return out
+
+    def _component_jacobian(self, c, r, window):
+        """_model_methods.py:260-396 for one component over ``window``."""
+        w = c.window & window
+        if max(w.nx, w.ny) > c.image_chunksize:
+            # _chunk_image_jacobian (_model_methods.py:350-396)
+            ncx = int(math.ceil(w.nx / c.image_chunksize))
+            ncy = int(math.ceil(w.ny / c.image_chunksize))
+            csx, csy = int(round(w.nx / ncx)), int(round(w.ny / ncy))
+            J = torch.zeros((w.ny, w.nx, len(r)), dtype=F64)
+            for nx in range(ncx):
+                for ny in range(ncy):
+                    sub = w.copy().crop_to_pixel(
+                        [[csx * nx, min(w.nx, csx * (nx + 1))], [csy * ny, min(w.ny, csy * (ny + 1))]])
+                    Js, gs = self._component_jacobian(c, r, sub)
+                    di = Grid.indices(w, gs)
+                    si = Grid.indices(gs, w)
+                    J[di] += Js[si]
+            return J, w
+        cols = []
+        for k in range(len(r)):
+            with fwAD.dual_level():
+                t = torch.zeros_like(r)
+                t[k] = 1.0
+                dual = fwAD.make_dual(r.clone(), t)
+                data, g = component_sample(c, c.rep_to_values(dual), sample_window=w)
+                cols.append(fwAD.unpack_dual(data).tangent.clone())
+        return torch.stack(cols, dim=-1), g
+
+    def jacobian(self, rep):
+        """model.jacobian(parameters=rep, as_representation=True).data: (H, W, Np)."""
+        J = torch.zeros((self.window.ny, self.window.nx, self.nparams), dtype=F64)
+        k = 0
+        for c, r in zip(self.components, self.split(rep)):
+            n = len(r)
+            if n:
+                Jc, g = self._component_jacobian(c, r.detach(), self.window)
+                di = Grid.indices(self.window, g)
+                si = Grid.indices(g, self.window)
+                J[di + (slice(k, k + n),)] += Jc[si]
+            k += n
+        return J
+
+
+# ---------------------------------------------------------------------------
+# chi^2 and Levenberg-Marquardt: utils/optimization.py:6-30, fit/lm.py:155-493
+# ---------------------------------------------------------------------------
+def chi_squared(target, model, mask=None, variance=None):
+    """utils/optimization.py:6-17."""
+    if mask is None:
+        if variance is None:
+            return torch.sum((target - model) ** 2)
+        return torch.sum(((target - model) ** 2) / variance)
+    keep = torch.logical_not(mask)
+    if variance is None:
+        return torch.sum((target[keep] - model[keep]) ** 2)
+    return torch.sum(((target[keep] - model[keep]) ** 2) / variance[keep])
+
+
+def reduced_chi_squared(target, model, params, mask=None, variance=None):
+    """utils/optimization.py:20-30."""
+    if mask is None:
+        ndf = len(target) - params
+    else:
+        ndf = torch.sum(torch.logical_not(mask)) - params
+    return chi_squared(target, model, mask, variance) / ndf
+
+
+class OptimizeStop(Exception):
+    """errors/fit.py."""
+
+
+def lm_damped_solve(L, grad, hess):
+    """fit/lm.py:359-371."""
+    I = torch.eye(len(grad), dtype=grad.dtype)
+    D = torch.ones_like(hess) - I
+    return torch.linalg.solve(hess * (I + D / (1 + L)) + L * I * (1 + torch.diag(hess)), grad)
+
+
+class LM:
+    """fit/lm.py:15-493 restated over an oracle ``Model``."""
+
+    def __init__(self, model: Model, data, weight=None, mask=None, initial_state=None, max_iter=100,
+                 relative_tolerance=1e-5, ndf=None, **kw):
+        self.model = model
+        self.current_state = (model.vector_representation() if initial_state is None
+                              else torch.as_tensor(initial_state, dtype=F64)).clone()
+        self.max_iter = max_iter
+        self.relative_tolerance = relative_tolerance
+        self.max_step_iter = kw.get("max_step_iter", 10)
+        self.curvature_limit = kw.get("curvature_limit", 1.0)
+        self._Lup = kw.get("Lup", 11.0)
+        self._Ldn = kw.get("Ldn", 9.0)
+        self.L = kw.get("L0", 1.0)
+        self.acceleration = kw.get("acceleration", 0.0)
+        self.Y = torch.as_tensor(data, dtype=F64).reshape(-1)
+        self.W = (torch.ones_like(self.Y) if weight is None
+                  else torch.as_tensor(weight, dtype=F64).reshape(-1))
+        self.mask = None if mask is None else torch.logical_not(torch.as_tensor(mask).reshape(-1))
+        if self.mask is not None and torch.sum(self.mask).item() == 0:
+            raise OptimizeStop("No data to fit. All pixels are masked")
+        if ndf is None:
+            n = len(self.Y) if self.mask is None else torch.sum(self.mask).item()
+            self.ndf = max(1.0, n - len(self.current_state))
+        else:
+            self.ndf = ndf
+        self.message = ""
+        self.n_forward = 0
+        self.n_jacobian = 0
+
+    def forward(self, rep):
+        self.n_forward += 1
+        return self.model.forward(rep).reshape(-1)
+
+    def jacobian(self, rep):
+        self.n_jacobian += 1
+        return self.model.jacobian(rep).reshape(-1, self.model.nparams)
+
+    def Lup(self):
+        self.L = min(1e9, self.L * self._Lup)
+
+    def Ldn(self):
+        self.L = max(1e-9, self.L / self._Ldn)
+
+    def _m(self, x):
+        return x if self.mask is None else x[self.mask]
+
+    def _chi2(self, Ypred):
+        return torch.sum(self._m(self.W * (self.Y - Ypred) ** 2)) / self.ndf
+
+    def _r(self, Y, Ypred, W):
+        return self._m(W) * (self._m(Y) - self._m(Ypred))
+
+    def step(self, chi2):
+        """fit/lm.py:248-357."""
+        Y0 = self.forward(self.current_state)
+        J = self.jacobian(self.current_state)
+        r = self._r(Y0, self.Y, self.W)
+        Jm = self._m(J)
+        Wm = self._m(self.W)
+        self.hess = Jm.T @ (Wm.view(-1, 1) * Jm)
+        self.grad = -Jm.T @ r
+        init_chi2 = chi2
+        nostep = True
+        best = (torch.zeros_like(self.current_state), init_chi2, self.L)
+        scarry_best = (None, init_chi2, self.L)
+        direction = "none"
+        d = 0.1
+        for iteration in range(self.max_step_iter):
+            if iteration > self.max_step_iter / 2 and self.L < 1e-3:
+                self.L = 1.0
+            h = lm_damped_solve(self.L, self.grad, self.hess)
+            Y1 = self.forward(self.current_state + d * h)
+            rh = self._r(Y1, self.Y, self.W)
+            rpp = Jm.T @ ((2 / d) * ((rh - r) / d - Wm * (Jm @ h)))
+            if self.L > 1e-4:
+                a = -lm_damped_solve(self.L, rpp, self.hess) / 2
+            else:
+                a = torch.zeros_like(h)
+            ha = h + a * self.acceleration
+            Y1 = self.forward(self.current_state + ha)
+            chi2 = self._chi2(Y1).item()
+            if not np.isfinite(chi2):
+                self.Lup()
+                if direction == "better":
+                    break
+                direction = "worse"
+                continue
+            if chi2 <= scarry_best[1]:
+                scarry_best = (ha, chi2, self.L)
+            rho = torch.linalg.norm(a) / torch.linalg.norm(h)
+            if rho > self.curvature_limit:
+                self.Lup()
+                if direction == "better":
+                    break
+                direction = "worse"
+                continue
+            if chi2 < best[1]:
+                best = (ha, chi2, self.L)
+                nostep = False
+                self.Ldn()
+                if self.L <= 1e-8 or direction == "worse":
+                    break
+                direction = "better"
+            elif chi2 > best[1] and direction in ["none", "worse"]:
+                self.Lup()
+                if self.L == 1e9:
+                    break
+                direction = "worse"
+            else:
+                break
+            if (best[1] - init_chi2) / init_chi2 < -0.1:
+                break
+        if nostep:
+            if scarry_best[0] is not None:
+                return scarry_best
+            raise OptimizeStop("Could not find step to improve chi^2")
+        return best
+
+    def fit(self):
+        """fit/lm.py:427-493."""
+        if len(self.current_state) == 0:
+            return self
+        self.loss_history = [self._chi2(self.forward(self.current_state)).item()]
+        self.L_history = [self.L]
+        self.lambda_history = [self.current_state.detach().clone().numpy()]
+        for iteration in range(self.max_iter):
+            try:
+                res = self.step(chi2=self.loss_history[-1])
+            except OptimizeStop:
+                self.message = self.message + "fail. Could not find step to improve Chi^2"
+                break
+            self.L = res[2]
+            self.current_state = (self.current_state + res[0]).detach()
+            self.L_history.append(self.L)
+            self.loss_history.append(res[1])
+            self.lambda_history.append(self.current_state.detach().clone().numpy())
+            self.Ldn()
+            if len(self.loss_history) >= 3:
+                if (self.loss_history[-3] - self.loss_history[-1]) / self.loss_history[-1] < \
+                        self.relative_tolerance and self.L < 0.1:
+                    self.message = self.message + "success"
+                    break
+            if len(self.loss_history) > 10:
+                if (self.loss_history[-10] - self.loss_history[-1]) / self.loss_history[-1] < \
+                        self.relative_tolerance:
+                    self.message = self.message + "success by immobility. Convergence not guaranteed"
+                    break
+        else:
+            self.message = self.message + "fail. Maximum iterations"
+        return self
+
+    def res(self):
+        """fit/base.py:104-117: the state with the lowest chi^2 in the history."""
+        N = np.isfinite(self.loss_history)
+        if np.sum(N) == 0:
+            return self.current_state
+        idx = int(np.argmin(np.array(self.loss_history)[N]))
+        return torch.as_tensor(np.array(self.lambda_history)[N][idx], dtype=F64)
+
+
+# ---------------------------------------------------------------------------
+# fixture PSFs: utils/initialize/construct_psf.py:8-65
+# ---------------------------------------------------------------------------
+def gaussian_psf(sigma, img_width, pixelscale, upsample=4, normalize=True):
+    """construct_psf.py:8-35."""
+    assert img_width % 2 == 1
+    N = img_width * upsample
+    g = np.linspace(-(N - 1) * pixelscale / (2 * upsample), (N - 1) * pixelscale / (2 * upsample), N)
+    XX, YY = np.meshgrid(g, g)
+    ZZ = np.exp(-0.5 * (XX**2 + YY**2) / sigma**2)
+    ZZ = ZZ.reshape(img_width, upsample, img_width, upsample).sum(axis=(1, 3)) / (upsample**2)
+    return ZZ / np.sum(ZZ) if normalize else ZZ
+
+
+def moffat_psf(n, Rd, img_width, pixelscale, upsample=4, normalize=True):
+    """construct_psf.py:38-65."""
+    assert img_width % 2 == 1
+    N = img_width * upsample
+    g = np.linspace(-(N - 1) * pixelscale / (2 * upsample), (N - 1) * pixelscale / (2 * upsample), N)
+    XX, YY = np.meshgrid(g, g)
+    ZZ = 1.0 / (1.0 + (XX**2 + YY**2) / (Rd**2)) ** n
+    ZZ = ZZ.reshape(img_width, upsample, img_width, upsample).sum(axis=(1, 3)) / (upsample**2)
+    return ZZ / np.sum(ZZ) if normalize else ZZ

@@ -1,0 +1,130 @@
+"""Parity of the gfx950 path with the float64 oracle on identical inputs.
+
+Tolerances (max-norm relative to the oracle's max):
+  model image  1e-5   (float32 render vs float64 oracle)
+  Jacobian     2e-4 per column (float32 analytic partials vs float64 forward AD)
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import astrophot_amd as ap
+from oracle import hotpath as H
+from tests.pairs import GAL, component, rel_err, single, target
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-5
+JAC_TOL = 2e-4
+
+PSF11 = H.gaussian_psf(1.5, 11, 1.0)
+
+
+@pytest.mark.parametrize("kind", ["sersic", "exponential", "gaussian", "moffat", "flat_sky"])
+@pytest.mark.parametrize("integrate", ["none", "threshold"])
+def test_render_psf_none(kind, integrate):
+    om, m = single(kind, GAL[kind], integrate_mode=integrate)
+    ref = om.forward(om.vector_representation())
+    got = m().data
+    assert got.is_cuda
+    assert rel_err(got, ref) < IMG_TOL
+
+
+@pytest.mark.parametrize("kind", ["sersic", "exponential", "gaussian", "moffat"])
+@pytest.mark.parametrize("integrate", ["none", "threshold"])
+def test_render_psf_full(kind, integrate):
+    om, m = single(kind, GAL[kind], psf=PSF11, psf_mode="full", integrate_mode=integrate)
+    ref = om.forward(om.vector_representation())
+    got = m().data
+    assert rel_err(got, ref) < IMG_TOL
+
+
+@pytest.mark.parametrize("kind", ["sersic", "exponential", "gaussian", "moffat", "flat_sky"])
+@pytest.mark.parametrize("psf_mode", ["none", "full"])
+def test_jacobian(kind, psf_mode):
+    if kind == "flat_sky" and psf_mode == "full":
+        pytest.skip("sky models never convolve (sky_model_object.py:21-27)")
+    kw = dict(psf=PSF11, psf_mode="full") if psf_mode == "full" else {}
+    om, m = single(kind, GAL[kind], **kw)
+    rep = om.vector_representation()
+    Jref = om.jacobian(rep)
+    J = m.jacobian(parameters=rep.clone(), as_representation=True).data
+    assert J.shape == Jref.shape
+    for k in range(J.shape[-1]):
+        assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k
+
+
+def test_sersic_quad_sampling():
+    """sampling_mode quad:5 (the reference fixtures' truth render, tests/utils.py:64)."""
+    om, m = single("sersic", GAL["sersic"], psf=PSF11, psf_mode="full", sampling_mode="quad:5")
+    ref = om.forward(om.vector_representation())
+    assert rel_err(m().data, ref) < IMG_TOL
+
+
+def test_group_c0_like():
+    """Sersic + flat sky group with a 25x25 Gaussian PSF (the 100x100 C0 setup at 64x64)."""
+    nx = ny = 64
+    psf = H.gaussian_psf(1.5, 25, 1.0)
+    tgt = target(nx, ny, psf=psf)
+    p = {"center": [32.3, 31.7], "q": 0.7, "PA": 0.6, "n": 2.0, "Re": 8.0, "Ie": 1.0}
+    oc, mc = component("sersic", p, tgt, nx, ny, psf=psf, psf_mode="full")
+    os_, ms = component("flat_sky", {"F": math.log10(0.5)}, tgt, nx, ny)
+    om = H.Model([oc, os_], window=H.Grid(nx, ny))
+    g = ap.models.AstroPhot_Model(name="grp", model_type="group model", models=[mc, ms], target=tgt)
+    rep = om.vector_representation()
+    assert np.allclose(rep.numpy(), g.parameters.vector_representation().numpy())
+    assert rel_err(g().data, om.forward(rep)) < IMG_TOL
+    J = g.jacobian(parameters=rep.clone(), as_representation=True).data
+    Jref = om.jacobian(rep)
+    for k in range(J.shape[-1]):
+        assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k
+
+
+def test_chi2_and_normal_equations():
+    lib = ap._lib.load()
+    torch.manual_seed(0)
+    n, npar = 5000, 7
+    dev = "cuda"
+    J = torch.randn(n, npar, device=dev, dtype=torch.float32)
+    m = torch.rand(n, device=dev)
+    y = torch.rand(n, device=dev)
+    w = torch.rand(n, device=dev) + 0.5
+    mask = (torch.rand(n, device=dev) < 0.1).to(torch.uint8)
+    wsb = lib.apx_reduce_workspace_bytes(n, npar)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    out = torch.zeros(1, dtype=torch.float64, device=dev)
+    st = ap._lib.stream()
+    ap._lib.check(lib.apx_chi2(m.data_ptr(), y.data_ptr(), w.data_ptr(), mask.data_ptr(), n, out.data_ptr(),
+                               ws.data_ptr(), wsb, st))
+    keep = mask == 0
+    ref = (w.double() * (y.double() - m.double()) ** 2)[keep].sum()
+    assert abs(out.item() - ref.item()) <= 1e-9 * abs(ref.item())
+    hess = torch.zeros(npar, npar, dtype=torch.float64, device=dev)
+    grad = torch.zeros(npar, dtype=torch.float64, device=dev)
+    ap._lib.check(lib.apx_normal_equations(J.data_ptr(), m.data_ptr(), y.data_ptr(), w.data_ptr(), mask.data_ptr(),
+                                           n, npar, hess.data_ptr(), grad.data_ptr(), ws.data_ptr(), wsb, st))
+    Jd, wd = J.double()[keep], w.double()[keep]
+    Href = Jd.T @ (wd[:, None] * Jd)
+    gref = Jd.T @ (wd * (y.double() - m.double())[keep])
+    assert torch.allclose(hess, Href, rtol=1e-5, atol=1e-5 * Href.abs().max().item())
+    assert torch.allclose(grad, gref, rtol=1e-5, atol=1e-5 * gref.abs().max().item())
+
+
+def test_lm_solve_matches_reference_formula():
+    lib = ap._lib.load()
+    torch.manual_seed(1)
+    npar = 8
+    A = torch.randn(40, npar, dtype=torch.float64)
+    hess = (A.T @ A).cuda()
+    grad = torch.randn(npar, dtype=torch.float64).cuda()
+    for Lval in (1e-9, 1e-3, 1.0, 1e4):
+        h = torch.empty_like(grad)
+        st = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ap._lib.check(lib.apx_lm_solve(hess.data_ptr(), grad.data_ptr(), npar, Lval, h.data_ptr(), st.data_ptr(),
+                                       ap._lib.stream()))
+        ref = H.lm_damped_solve(Lval, grad.cpu(), hess.cpu())
+        assert st.item() == 0
+        assert torch.allclose(h.cpu(), ref, rtol=1e-9, atol=1e-12)

@@ -1,0 +1,192 @@
+"""Host-side checks that need no GPU: the C-ABI library, window geometry,
+parameter vectors and the launch descriptors the engine builds."""
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import astrophot_amd as ap
+from astrophot_amd import _lib as L
+from oracle import hotpath as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "astrophot_hip.h")
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(apx_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert L.LIB_PATH.exists(), "run python -m astrophot_amd._build"
+    lib = ctypes.CDLL(str(L.LIB_PATH))
+    syms = header_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert sorted(L.SIGNATURES) == syms
+
+
+def test_struct_layouts_match_header():
+    """ctypes mirrors of apx_component / apx_conv_job list every header field in order."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for cname, cls in (("apx_component", L.Component), ("apx_conv_job", L.ConvJob)):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), text, re.S).group(1)
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            names = decl.split(None, 1)[1].replace("*", " ")
+            for n in names.split(","):
+                fields.append(re.sub(r"\[.*\]", "", n).strip())
+        assert [f[0] for f in cls._fields_] == fields, cname
+
+
+def test_library_workspace_queries_without_gpu():
+    lib = ctypes.CDLL(str(L.LIB_PATH))
+    lib.apx_render_workspace_bytes.restype = ctypes.c_int64
+    lib.apx_render_workspace_bytes.argtypes = [ctypes.c_int64]
+    assert lib.apx_render_workspace_bytes(1000) >= 8 * 1000
+
+
+def test_product_fails_loudly_without_gpu():
+    if torch.cuda.is_available():
+        pytest.skip("GPU visible")
+    with pytest.raises(ap.errors.APXError):
+        L.load()
+
+
+@pytest.mark.parametrize("P", [1.0, 0.8, [[0.9, 0.1], [-0.05, 1.1]]])
+def test_window_geometry_matches_oracle(P):
+    w = ap.image.Window(pixel_shape=(40, 30), pixelscale=P)
+    g = H.Grid(40, 30, P)
+    assert torch.allclose(w.origin, g.origin) and torch.allclose(w.end, g.end)
+    xy = torch.tensor([3.3, -1.7], dtype=torch.float64)
+    assert torch.allclose(w.plane_to_pixel(xy), g.plane_to_pixel(xy))
+    assert torch.allclose(w.get_coordinate_meshgrid(), g.meshgrid())
+    # pad / crop keep the plane position of pixels
+    w2 = w.copy().pad_pixel((5, 4))
+    g2 = g.copy().pad(5, 4)
+    assert (w2.nx, w2.ny) == (g2.nx, g2.ny) and torch.allclose(w2.reference_imageij, g2.ref_ij)
+    # intersection / union / indices (window_object.py:241-253, 483-523)
+    o = ap.image.Window(pixel_shape=(12, 9), origin=w.pixel_to_plane(torch.tensor([30.5, 20.5])), pixelscale=P)
+    og = H.Grid(12, 9, P, ref_xy=g.pixel_to_plane(torch.tensor([30.5, 20.5], dtype=torch.float64)))
+    a, ag = w & o, g & og
+    assert (a.nx, a.ny) == (ag.nx, ag.ny) and torch.allclose(a.origin, ag.origin)
+    u, ug = w | o, g | og
+    assert (u.nx, u.ny) == (ug.nx, ug.ny) and torch.allclose(u.origin, ug.origin)
+    assert w.get_self_indices(o) == H.Grid.indices(g, og)
+
+
+def test_window_slicing_kat():
+    """test_window.py:87-103 style: a sub-window's indices into its parent."""
+    w = ap.image.Window(pixel_shape=(100, 100), pixelscale=1.0)
+    sub = w.copy().crop_to_pixel([[10, 30], [20, 45]])
+    assert w.get_self_indices(sub) == (slice(20, 45), slice(10, 30))
+
+
+def test_psf_border_and_odd_shape():
+    """test_image.py:412-436: border of a 9x9 PSF is 5; even PSFs are rejected."""
+    p = ap.image.PSF_Image(data=np.ones((9, 9)), pixelscale=1.0)
+    assert p.psf_border_int == (5, 5)
+    with pytest.raises(ap.errors.SpecificationConflict):
+        ap.image.PSF_Image(data=np.ones((8, 9)), pixelscale=1.0)
+
+
+def test_reduce_sums_blocks():
+    """test_image.py:438-459."""
+    img = ap.image.Target_Image(data=np.ones((30, 36)), pixelscale=1.0)
+    small = img.reduce(3)
+    assert tuple(small.data.shape) == (10, 12) and float(small.data[0, 0]) == 9.0
+
+
+def test_parameter_kats():
+    """test_parameter.py:117-161."""
+    P = ap.param.Parameter_Node("low", limits=(1, None))
+    P.value = 100.0
+    with pytest.raises(ap.errors.InvalidParameter):
+        P.value = -100.0
+    P = ap.param.Parameter_Node("cyc", limits=(-1, 1), cyclic=True)
+    P.value = 2.0
+    assert P.value.item() == 0.0
+    P.value = -2.0
+    assert P.value.item() == 0.0
+
+
+def test_parameter_vector_round_trip_masked():
+    """test_parameter.py:379-418 (without the functional node)."""
+    P1 = ap.param.Parameter_Node("t1", value=0.5, limits=(-1, 1))
+    P2 = ap.param.Parameter_Node("t2", value=2.0)
+    P3 = ap.param.Parameter_Node("t3", value=[4.0, 5.0], limits=(1.0, None))
+    P6 = ap.param.Parameter_Node("t6", value=((5, 6), (7, 8)), limits=(None, 10.0))
+    G = ap.param.Parameter_Node("g")
+    for p in (P1, P2, P3, P6):
+        G.link(p)
+    assert len(G.vector_representation()) == 8
+    mask = torch.tensor([1, 1, 0, 1, 0, 1, 0, 1], dtype=torch.bool)
+    with ap.param.Param_Mask(G, mask):
+        vec = G.vector_values().clone()
+        G.vector_set_representation(G.vector_representation())
+        assert torch.allclose(vec, G.vector_values(), rtol=1e-12)
+        assert G.vector_values().numel() == 5
+
+
+def test_model_parameter_order_and_reps_match_oracle():
+    tgt = ap.image.Target_Image(data=np.zeros((36, 40)), pixelscale=1.0)
+    p = {"center": [20.3, 17.6], "q": 0.6, "PA": 3.5, "n": 2.5, "Re": 6.0, "Ie": 1.0}
+    m = ap.models.AstroPhot_Model(name="s", model_type="sersic galaxy model", target=tgt, parameters=p)
+    oc = H.Component("sersic", H.Grid(40, 36), p)
+    # PA wraps into [0, pi) on set (parameter.py:436-440)
+    assert m["PA"].value.item() == pytest.approx(3.5 - math.pi)
+    rep = m.parameters.vector_representation()
+    assert torch.allclose(rep, oc.val_to_rep(oc.vector_values()).reshape(-1) if False else
+                          H.Model([H.Component("sersic", H.Grid(40, 36), dict(p, PA=3.5 - math.pi))]).vector_representation())
+    assert list(m.parameters.vector_names()) == ["center:0", "center:1", "q", "PA", "n", "Re", "Ie"]
+    with pytest.raises(ap.errors.UnrecognizedModel):
+        ap.models.AstroPhot_Model(name="x", model_type="nonexistent model", target=tgt)
+
+
+class _FakeLib:
+    def __init__(self):
+        self.calls = []
+
+    def __getattr__(self, name):
+        def f(*a):
+            self.calls.append((name, a))
+            return 1 << 12 if name.endswith("workspace_bytes") else 0
+        return f
+
+
+def test_engine_descriptors(monkeypatch):
+    """Geometry handed to the render kernel equals the oracle's working grid."""
+    fake = _FakeLib()
+    monkeypatch.setattr(L, "load", lambda require_gpu=True: fake)
+    monkeypatch.setattr(L, "stream", lambda: 0)
+    psf = H.gaussian_psf(1.5, 11, 1.0)
+    tgt = ap.image.Target_Image(data=np.zeros((36, 40)), pixelscale=0.8, psf=ap.image.PSF_Image(data=psf, pixelscale=0.8))
+    p = {"center": [16.3, 14.1], "q": 0.6, "PA": 0.7, "n": 2.5, "Re": 6.0, "Ie": 1.0}
+    m = ap.models.AstroPhot_Model(name="s", model_type="sersic galaxy model", target=tgt, parameters=p,
+                                  psf_mode="full")
+    m()
+    comp = [a for n, a in fake.calls if n == "apx_render_component"][0][0]._obj
+    g = H.Grid(40, 36, 0.8).pad(6, 6)  # 11x11 PSF: border ceil(12/2) = 6
+    pc = g.plane_to_pixel(torch.tensor(p["center"], dtype=torch.float64))
+    assert (comp.nx, comp.ny) == (g.nx, g.ny)
+    assert (comp.ci, comp.cj) == tuple(int(v) for v in torch.round(pc))
+    assert comp.frac[0] == 0.0 and comp.frac[1] == 0.0
+    F = H.sersic_Ie_to_flux(torch.tensor(10.0, dtype=torch.float64), torch.tensor(2.5, dtype=torch.float64), 6.0, 0.6).item()
+    assert comp.reference == pytest.approx(F / (g.nx * g.ny), rel=1e-12)
+    assert comp.pixel_area == pytest.approx(0.64)
+    shift = [a for n, a in fake.calls if n == "apx_psf_shift"][0]
+    assert shift[3] == pytest.approx(float(pc[0] - torch.round(pc[0])))
+    assert shift[4] == pytest.approx(float(pc[1] - torch.round(pc[1])))
+    conv = [a for n, a in fake.calls if n == "apx_convolve"][0][0]._obj
+    assert (conv.x0, conv.y0, conv.ow, conv.oh) == (6, 6, 40, 36)

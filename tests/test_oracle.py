@@ -1,0 +1,303 @@
+"""Pins of the float64 oracle (CPU only).
+
+Each test checks an oracle function against something that does not restate
+it: the reference's own known-answer tests, golden vectors written by running
+the reference's utils/operations.py, library routines (scipy, numpy), closed
+forms, invariants and brute force on tiny inputs.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+from scipy.signal import fftconvolve
+from scipy.special import erf, gamma, roots_legendre
+
+from oracle import hotpath as H
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "operations_vectors.npz"))
+F64 = torch.float64
+
+
+# -- reference KATs: test_utils.py:344-434 (Sersic conversions) ---------------
+def test_sersic_flux_kat():
+    b1 = H.sersic_n_to_b(1.0)
+    got = H.sersic_Ie_to_flux(torch.tensor(1.0, dtype=F64), torch.tensor(1.0, dtype=F64), 1.0, 1.0)
+    assert got.item() == pytest.approx(2 * np.pi * gamma(2) * np.exp(b1) * b1 ** (-2), rel=1e-12)
+
+
+@pytest.mark.parametrize("n", [0.7, 1.0, 2.5, 4.0])
+def test_sersic_flux_brute_force(n):
+    """Integral of the profile over a fine polar grid equals the total flux."""
+    Re, q = 2.0, 0.6
+    r = np.geomspace(1e-6, 400 * Re, 400001)
+    I = H.sersic(torch.tensor(r), torch.tensor(n), torch.tensor(Re), torch.tensor(1.0)).numpy()
+    flux = 2 * np.pi * q * np.trapezoid(I * r, r)  # elliptical: area element scales by q
+    F = H.sersic_Ie_to_flux(torch.tensor(1.0, dtype=F64), torch.tensor(n, dtype=F64), Re, q).item()
+    assert flux == pytest.approx(F, rel=2e-4)
+
+
+def test_b_n_half_light():
+    """b(n) makes Re the half-light radius: incomplete gamma P(2n, b) = 1/2."""
+    from scipy.special import gammainc
+    for n in (0.5, 1.0, 2.0, 4.0, 8.0):
+        assert gammainc(2 * n, H.sersic_n_to_b(n)) == pytest.approx(0.5, abs=2e-4)
+
+
+def test_profile_closed_forms():
+    r = np.geomspace(1e-8, 2000.0, 400001)
+    # gaussian with the 1-D normalisation quirk: integral = I0 * sqrt(2 pi) * sigma
+    s = 1.7
+    I = H.gaussian(torch.tensor(r), torch.tensor(s), torch.tensor(1.0)).numpy()
+    assert 2 * np.pi * np.trapezoid(I * r, r) == pytest.approx(np.sqrt(2 * np.pi) * s, rel=1e-6)
+    # moffat: integral = I0 pi Rd^2 / (n - 1)   (conversions/functions.py moffat_I0_to_flux)
+    n, Rd = 3.0, 2.2
+    I = H.moffat(torch.tensor(r), torch.tensor(n), torch.tensor(Rd), torch.tensor(1.0)).numpy()
+    assert 2 * np.pi * np.trapezoid(I * r, r) == pytest.approx(np.pi * Rd**2 / (n - 1), rel=1e-5)
+    # exponential is the Sersic n = 1 profile
+    R = torch.linspace(0.01, 30, 200, dtype=F64)
+    e = H.exponential(R, torch.tensor(3.0, dtype=F64), torch.tensor(2.0, dtype=F64))
+    s1 = H.sersic(R, torch.tensor(1.0, dtype=F64), torch.tensor(3.0, dtype=F64), torch.tensor(2.0, dtype=F64))
+    assert torch.allclose(e, s1, rtol=1e-14)
+
+
+# -- parameter maps: test_parameter.py:117-161, 379-418 ------------------------
+def test_boundaries_round_trip_and_kats():
+    v = torch.tensor([0.2, 0.5, 0.93], dtype=F64)
+    assert torch.allclose(H.inv_boundaries(H.boundaries(v, 0.0, 1.0), 0.0, 1.0), v, rtol=1e-14)
+    v = torch.tensor([1.5, 4.0, 100.0], dtype=F64)
+    assert torch.allclose(H.inv_boundaries(H.boundaries(v, 1.0, None), 1.0, None), v, rtol=1e-13)
+    v = torch.tensor([-100.0, -3.0, 0.5], dtype=F64)
+    assert torch.allclose(H.inv_boundaries(H.boundaries(v, None, 1.0), None, 1.0), v, rtol=1e-13)
+    # cyclic wrap of (-1, 1): 2 -> 0 and -2 -> 0 (test_parameter.py:146-161)
+    assert H.cyclic_boundaries(torch.tensor(2.0), -1.0, 1.0).item() == 0.0
+    assert H.cyclic_boundaries(torch.tensor(-2.0), -1.0, 1.0).item() == 0.0
+
+
+def test_boundaries_derivatives_closed_form():
+    """optimization.py:40-47 closed forms vs autograd of inv_boundaries."""
+    r = torch.linspace(-5, 5, 41, dtype=F64, requires_grad=True)
+    for lo, hi, f in [(0.0, 1.0, lambda r: (1.0 - 0.0) / (np.pi * (r**2 + 1))),
+                      (0.0, None, lambda r: 0.5 + 0.5 * (r - 0.0) / torch.sqrt((r - 0.0) ** 2 + 4)),
+                      (None, 2.0, lambda r: 0.5 - 0.5 * (r - 2.0) / torch.sqrt((r - 2.0) ** 2 + 4))]:
+        g, = torch.autograd.grad(H.inv_boundaries(r, lo, hi).sum(), r)
+        assert torch.allclose(g, f(r.detach()), rtol=1e-12)
+
+
+# -- quadrature: operations.py:94-247 -----------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7])
+def test_quad_table_golden(n):
+    P = torch.tensor(GOLD["P"])
+    x, y, w = H.quad_table(n, P)
+    assert np.allclose(x.numpy(), GOLD[f"qt{n}_x"], rtol=0, atol=1e-15)
+    assert np.allclose(y.numpy(), GOLD[f"qt{n}_y"], rtol=0, atol=1e-15)
+    assert np.allclose(w.numpy(), GOLD[f"qt{n}_w"], rtol=0, atol=1e-15)
+    # numpy's leggauss (what the HIP path's host side passes to the kernels)
+    # gives the table scipy's roots_legendre gives the reference
+    a, wt = np.polynomial.legendre.leggauss(n)
+    sa, sw = roots_legendre(n)
+    assert np.allclose(a, sa, atol=1e-15) and np.allclose(wt, sw, atol=1e-15)
+    assert np.isclose(w.sum().item(), 1.0, atol=1e-14)
+
+
+def test_quad_exact_on_polynomials():
+    """3x3 Gauss-Legendre integrates x^a y^b (a, b <= 5) over a unit pixel exactly."""
+    P = torch.eye(2, dtype=F64)
+    for a in range(6):
+        for b in range(6):
+            f = lambda X, Y, P: X**a * Y**b
+            res, _ = H.single_quad_integrate(torch.zeros(1, dtype=F64), torch.zeros(1, dtype=F64), P, f, 3)
+            exact = ((0.5 ** (a + 1) - (-0.5) ** (a + 1)) / (a + 1)) * ((0.5 ** (b + 1) - (-0.5) ** (b + 1)) / (b + 1))
+            assert res.item() == pytest.approx(exact, abs=1e-15)
+
+
+def _gold_brightness(X, Y, P):
+    n, Re, Ie, q, PA, soft = GOLD["params"]
+    th = -(PA - math.pi / 2)
+    c, s = math.cos(th), math.sin(th)
+    Xr, Yr = c * X - s * Y, (s * X + c * Y) / q
+    R = torch.sqrt(Xr**2 + Yr**2 + soft**2)
+    area = torch.linalg.det(P).abs()
+    return area * 10**Ie * torch.exp(-H.sersic_n_to_b(n) * (torch.pow(R / Re, 1 / n) - 1))
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+@pytest.mark.parametrize("ref", [1e-3, 1e-2])
+def test_grid_integrate_golden(depth, ref):
+    """Vectors written by the reference's own grid_integrate."""
+    X, Y, P = (torch.tensor(GOLD[k]) for k in ("X", "Y", "P"))
+    got = H.grid_integrate(X, Y, P, _gold_brightness, 3, 5, 1, depth, ref)
+    assert np.allclose(got.numpy(), GOLD[f"grid_d{depth}_r{ref:g}"], rtol=1e-13, atol=0)
+
+
+def test_single_quad_golden():
+    X, Y, P = (torch.tensor(GOLD[k]) for k in ("X", "Y", "P"))
+    res, refc = H.single_quad_integrate(X, Y, P, _gold_brightness, 5)
+    assert np.allclose(res.numpy(), GOLD["quad5_res"], rtol=1e-13)
+    assert np.allclose(refc.numpy(), GOLD["quad5_ref"], rtol=1e-13)
+
+
+def test_grid_integrate_gaussian_erf():
+    """Deep adaptive integration of a narrow round Gaussian matches the erf
+    closed form of its pixel integral."""
+    s = 0.6
+    P = torch.eye(2, dtype=F64)
+    f = lambda X, Y, P: torch.linalg.det(P).abs() * torch.exp(-0.5 * (X**2 + Y**2) / s**2) / (2 * np.pi * s**2)
+    xs = torch.tensor([0.0, 0.3, 0.8, -1.2], dtype=F64)
+    ys = torch.tensor([0.0, -0.4, 0.5, 0.9], dtype=F64)
+    got = H.grid_integrate(xs, ys, P, f, 3, 5, 1, 3, 1e-9)
+    ex = [0.25 * (erf((x + 0.5) / (s * np.sqrt(2))) - erf((x - 0.5) / (s * np.sqrt(2))))
+          * (erf((y + 0.5) / (s * np.sqrt(2))) - erf((y - 0.5) / (s * np.sqrt(2)))) for x, y in zip(xs.numpy(), ys.numpy())]
+    assert np.allclose(got.numpy(), ex, rtol=2e-7, atol=1e-12)
+
+
+# -- curvature, bilinear shift, convolution ------------------------------------
+def test_curvature_brute_force():
+    rng = np.random.default_rng(3)
+    m = rng.normal(size=(7, 9))
+    got = H.curvature(torch.tensor(m)).numpy()
+    for j in range(7):
+        for i in range(9):
+            jj, ii = min(max(j, 1), 5), min(max(i, 1), 7)
+            want = m[jj - 1, ii] + m[jj + 1, ii] + m[jj, ii - 1] + m[jj, ii + 1] - 4 * m[jj, ii]
+            assert got[j, i] == pytest.approx(want, abs=1e-12)
+
+
+@pytest.mark.parametrize("shift", [(0.3, -0.2), (-0.5, 0.5), (0.0, 0.0), (0.49, 0.01)])
+def test_shift_psf_brute_force(shift):
+    """interp2d with clamped corners, written out pixel by pixel."""
+    rng = np.random.default_rng(4)
+    psf = rng.uniform(0.1, 1.0, (5, 3))
+    got = H.shift_psf(torch.tensor(psf), torch.tensor(shift, dtype=F64)).numpy()
+    p = np.pad(psf, 1)
+    h, w = p.shape
+    for r in range(h):
+        for c in range(w):
+            x, y = c - shift[0], r - shift[1]
+            x0f, y0f = math.floor(x), math.floor(y)
+            x0, x1 = min(max(x0f, 0), w - 2), min(max(x0f + 1, 1), w - 1)
+            y0, y1 = min(max(y0f, 0), h - 2), min(max(y0f + 1, 1), h - 1)
+            want = (p[y0, x0] * (x1 - x) * (y1 - y) + p[y1, x0] * (x1 - x) * (y - y0)
+                    + p[y0, x1] * (x - x0) * (y1 - y) + p[y1, x1] * (x - x0) * (y - y0))
+            assert got[r, c] == pytest.approx(want, abs=1e-14)
+    if shift == (0.0, 0.0):
+        assert np.allclose(got[1:-1, 1:-1], psf)
+
+
+def test_fft_convolve_golden_and_scipy():
+    img, psf = torch.tensor(GOLD["fft_img"]), torch.tensor(GOLD["fft_psf"])
+    assert np.allclose(H.fft_convolve(img, psf).numpy(), GOLD["fft_prepadded"], rtol=1e-12, atol=1e-12)
+    # reference KAT (test_utils.py:15-38): interior equals scipy fftconvolve mode="same"
+    sp = fftconvolve(img.numpy(), psf.numpy(), mode="same")
+    got = H.fft_convolve(img, psf).numpy()
+    assert np.allclose(got[4:-4, 3:-3], sp[4:-4, 3:-3], rtol=1e-10, atol=1e-12)
+    # and the direct mode agrees on the interior (_model_methods.py:247-255)
+    d = H.direct_convolve(img, psf).numpy()
+    assert np.allclose(got[4:-4, 3:-3], d[4:-4, 3:-3], rtol=1e-10, atol=1e-12)
+
+
+# -- whole-component invariants -----------------------------------------------
+def _comp(kind="gaussian", center=(20.0, 18.0), psf=None, **kw):
+    p = {"gaussian": {"q": 0.999999, "PA": 0.4, "sigma": 2.0, "flux": 1.0},
+         "sersic": {"q": 0.6, "PA": 0.4, "n": 2.0, "Re": 4.0, "Ie": 0.0}}[kind]
+    p = dict(p, center=list(center))
+    return H.Component(kind, H.Grid(40, 36), p, psf=psf, **kw)
+
+
+def test_flux_conservation_under_psf():
+    """A unit-sum PSF moves flux but does not change its total (centre on a
+    pixel centre, so the sampling grids coincide)."""
+    psf = torch.tensor(H.gaussian_psf(1.2, 9, 1.0))
+    a = _comp(center=(20.5, 18.5), psf=psf, psf_mode="full", integrate_mode="none")
+    b = _comp(center=(20.5, 18.5), integrate_mode="none")
+    ia = H.Model([a]).forward(H.Model([a]).vector_representation())
+    ib = H.Model([b]).forward(H.Model([b]).vector_representation())
+    assert ia.sum().item() == pytest.approx(ib.sum().item(), rel=1e-9)
+
+
+def test_whole_pixel_translation():
+    psf = torch.tensor(H.gaussian_psf(1.2, 9, 1.0))
+    a = _comp("sersic", (18.3, 16.7), psf=psf, psf_mode="full")
+    b = _comp("sersic", (21.3, 18.7), psf=psf, psf_mode="full")
+    ia = H.Model([a]).forward(H.Model([a]).vector_representation())
+    ib = H.Model([b]).forward(H.Model([b]).vector_representation())
+    # selection threshold uses the same reference (same numel, same flux): images shift exactly
+    assert torch.allclose(ib[2 + 5:-5, 3 + 5:-5], ia[5:-5 - 2, 5:-5 - 3], rtol=1e-12, atol=1e-14)
+
+
+def test_round_profile_pa_independent_and_symmetric():
+    a = H.Component("sersic", H.Grid(31, 31), {"center": [15.5, 15.5], "q": 0.99999999, "PA": 0.3,
+                                               "n": 1.5, "Re": 4.0, "Ie": 0.0})
+    b = H.Component("sersic", H.Grid(31, 31), {"center": [15.5, 15.5], "q": 0.99999999, "PA": 2.1,
+                                               "n": 1.5, "Re": 4.0, "Ie": 0.0})
+    ia = H.Model([a]).forward(H.Model([a]).vector_representation())
+    ib = H.Model([b]).forward(H.Model([b]).vector_representation())
+    assert torch.allclose(ia, ib, rtol=1e-7)
+    assert torch.allclose(ia, ia.flip(0), rtol=1e-7)
+    assert torch.allclose(ia, ia.T, rtol=1e-7)
+
+
+def test_jacobian_vs_central_fd_and_reverse_mode():
+    psf = torch.tensor(H.gaussian_psf(1.2, 7, 1.0))
+    c = _comp("sersic", (20.3, 17.6), psf=psf, psf_mode="full")
+    m = H.Model([c])
+    rep = m.vector_representation()
+    J = m.jacobian(rep)
+    h = 1e-5
+    for k in range(len(rep)):
+        r1, r2 = rep.clone(), rep.clone()
+        r1[k] += h
+        r2[k] -= h
+        fd = (m.forward(r1) - m.forward(r2)) / (2 * h)
+        assert (fd - J[..., k]).abs().max() <= 1e-6 * J[..., k].abs().max() + 1e-12
+    # reverse mode on a projection
+    r = rep.clone().requires_grad_(True)
+    wgt = torch.linspace(0, 1, J.shape[0] * J.shape[1], dtype=F64).reshape(J.shape[:2])
+    g, = torch.autograd.grad((m.forward(r) * wgt).sum(), r)
+    assert torch.allclose(g, (J * wgt[..., None]).sum(dim=(0, 1)), rtol=1e-10)
+
+
+# -- chi^2 KATs: test_utils.py:55-121 ------------------------------------------
+def test_chi2_kats():
+    ones, zeros = torch.ones(10, dtype=F64), torch.zeros(10, dtype=F64)
+    mask = torch.zeros(10, dtype=torch.bool)
+    mask[2] = True
+    var = 2 * torch.ones(10, dtype=F64)
+    assert H.chi_squared(ones, zeros, mask=mask, variance=var).item() == 4.5
+    assert H.reduced_chi_squared(ones, zeros, 3, mask=mask, variance=var).item() == 0.75
+    assert H.chi_squared(ones, zeros, variance=var).item() == 5
+    assert H.reduced_chi_squared(ones, zeros, 3, variance=var).item() == pytest.approx(5 / 7)
+    assert H.chi_squared(ones, zeros, mask=mask).item() == 9
+    assert H.reduced_chi_squared(ones, zeros, 3, mask=mask).item() == 1.5
+    assert H.chi_squared(ones, zeros).item() == 10
+    assert H.reduced_chi_squared(ones, zeros, 3).item() == pytest.approx(10 / 7)
+
+
+# -- LM --------------------------------------------------------------------------
+def test_lm_damped_solve_numpy():
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=(30, 6))
+    Hm, g = A.T @ A, rng.normal(size=6)
+    for L in (1e-9, 0.1, 1.0, 1e5):
+        M = Hm / (1 + L)
+        np.fill_diagonal(M, np.diag(Hm) + L * (1 + np.diag(Hm)))
+        want = np.linalg.solve(M, g)
+        got = H.lm_damped_solve(L, torch.tensor(g), torch.tensor(Hm)).numpy()
+        assert np.allclose(got, want, rtol=1e-10)
+
+
+def test_lm_recovers_truth_noiseless():
+    psf = torch.tensor(H.gaussian_psf(1.2, 7, 1.0))
+    truth = {"center": [16.2, 15.7], "q": 0.7, "PA": 0.6, "n": 1.8, "Re": 4.0, "Ie": 1.0}
+    c = H.Component("sersic", H.Grid(32, 32), truth, psf=psf, psf_mode="full")
+    m = H.Model([c])
+    rep0 = m.vector_representation()
+    data = m.forward(rep0)
+    start = rep0.clone()
+    start[2:] *= 1.05
+    start[:2] += 0.2
+    lm = H.LM(m, data, weight=torch.ones_like(data) * 100.0, initial_state=start).fit()
+    assert lm.message.startswith("success")
+    assert torch.allclose(torch.as_tensor(lm.res()), rep0, rtol=1e-5, atol=1e-6)

@@ -22,6 +22,7 @@ working-grid pixel count (sersic_model.py:78-89) or the mean initial sample
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -74,9 +75,27 @@ class RenderRequest:
     jac: Optional[object] = None    # Jacobian_Image to accumulate columns into
     slot_cols: Optional[Dict[int, int]] = None  # slot -> Jacobian column
     dvdr: Optional[Dict[int, float]] = None     # slot -> d value / d rep
+    region: Optional[object] = None  # window the result is clipped to (default: image/jac window)
 
 
 _UNIT_KERNEL = {}
+
+# When a dict, every library launch below is bracketed by torch.cuda events on
+# the launch stream: TIMING[name] collects (start, end, work) triples, with
+# `work` the launch's algorithmic flops (convolve) or pixels (render).
+TIMING = None
+
+
+def _launch(name, work, fn):
+    if TIMING is None:
+        return fn()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    r = fn()
+    e.record()
+    TIMING.setdefault(name, []).append((s, e, work))
+    return r
 
 
 def _unit_kernel(device):
@@ -87,12 +106,31 @@ def _unit_kernel(device):
     return k
 
 
-def _box(dst_window, src_window):
-    """Rows/columns of ``src`` covering ``dst`` and where they land in ``dst``
+@functools.lru_cache(maxsize=16)
+def _gl(n):
+    x, w = gauss_legendre(n)
+    return tuple(float(v) for v in x), tuple(float(v) for v in w)
+
+
+def _buffer(owner, name, nbytes, device):
+    """A device byte buffer of at least ``nbytes`` kept on ``owner`` between
+    calls (work on one stream is ordered, so reuse is safe)."""
+    cache = owner.__dict__.setdefault("_apx_buffers", {})
+    b = cache.get(name)
+    if b is None or b.numel() < nbytes or b.device != torch.device(device):
+        b = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+        cache[name] = b
+    return b
+
+
+def _box(dst_window, src_window, region):
+    """Where ``region`` lands in ``dst`` and in ``src``, and its extent
     (image_object.py:450-457 via window_object.py:241-269)."""
-    dr, dc = src_window.get_other_indices(dst_window)  # into dst
-    sr, sc = dst_window.get_other_indices(src_window)  # into src
-    return (dr.start, dc.start), (sr.start, sc.start), (sr.stop - sr.start, sc.stop - sc.start)
+    dr, dc = dst_window.get_self_indices(region)
+    sr, sc = src_window.get_self_indices(region)
+    h = min(dr.stop - dr.start, sr.stop - sr.start)
+    w = min(dc.stop - dc.start, sc.stop - sc.start)
+    return (dr.start, dc.start), (sr.start, sc.start), (max(h, 0), max(w, 0))
 
 
 def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
@@ -102,8 +140,8 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     kind = model._profile
     psf_full = "full" in model.psf_mode and kind != "flat_sky"
     window = req.window.copy()
-    center = values["center"]
-    P = window.pixelscale
+    cx, cy = (float(v) for v in values["center"].reshape(-1).tolist())
+    P = window._P
     cp = L.Component()
     cp.profile = PROFILE_CODE[kind]
     # sampling mode (_model_methods.py:82-148)
@@ -115,9 +153,8 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
         cp.sample_quad = int(sm[sm.find(":") + 1:])
         if not 1 <= cp.sample_quad <= L.APX_MAX_QUAD_LEVEL:
             raise SpecificationConflict(f"sampling quad level must be 1..{L.APX_MAX_QUAD_LEVEL}")
-        x, w = gauss_legendre(cp.sample_quad)
-        for k in range(cp.sample_quad):
-            cp.sq_node[k], cp.sq_weight[k] = x[k], w[k]
+        x, w = _gl(cp.sample_quad)
+        cp.sq_node[:cp.sample_quad], cp.sq_weight[:cp.sample_quad] = x, w
     else:
         raise SpecificationConflict(
             f"{model.name} has unknown sampling mode: {sm}. Should be one of: midpoint, quad:level")
@@ -130,42 +167,40 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
         cp.quad_level = int(model.integrate_quad_level)
         cp.gridding = int(model.integrate_gridding)
         cp.max_depth = int(model.integrate_max_depth)
-        x, w = gauss_legendre(cp.quad_level)
-        for k in range(cp.quad_level):
-            cp.gl_node[k], cp.gl_weight[k] = x[k], w[k]
+        if not 1 <= cp.quad_level <= L.APX_MAX_QUAD_LEVEL:
+            raise SpecificationConflict(f"integrate_quad_level must be 1..{L.APX_MAX_QUAD_LEVEL}")
+        x, w = _gl(cp.quad_level)
+        cp.gl_node[:cp.quad_level], cp.gl_weight[:cp.quad_level] = x, w
 
     shift = None
     if psf_full:
         psf = model.psf
         if psf is None:
             raise SpecificationConflict(f"{model.name} has psf_mode 'full' but no PSF")
-        upscale = int(torch.round(window.pixel_length / psf.pixel_length))
-        if upscale != 1:
+        if round(math.sqrt(window._area / psf.window._area)) != 1:
             raise SpecificationConflict("PSF pixelscale must equal the image pixelscale (upscale 1)")
         bx, by = psf.psf_border_int
         window.pad_pixel((bx, by))
-        pc = window.plane_to_pixel(center)
+        pcx, pcy = window._x2p(cx, cy)
+        ci, cj = round(pcx), round(pcy)  # ties to even, as torch.round
         if model.psf_subpixel_shift == "none":
-            cij = torch.round(pc)
-            frac = P @ (cij - pc)
+            fx, fy = P[0] * (ci - pcx) + P[1] * (cj - pcy), P[2] * (ci - pcx) + P[3] * (cj - pcy)
         elif model.psf_subpixel_shift == "bilinear":
-            cij = torch.round(pc)
-            shift = pc - cij
-            frac = torch.zeros(2, dtype=torch.float64)
+            shift = (pcx - ci, pcy - cj)
+            fx = fy = 0.0
         else:
             raise SpecificationConflict(f"unrecognized subpixel shift method: {model.psf_subpixel_shift}")
     else:
         bx = by = 0
-        pc = window.plane_to_pixel(center)
-        cij = torch.round(pc)
-        frac = P @ (cij - pc)
+        pcx, pcy = window._x2p(cx, cy)
+        ci, cj = round(pcx), round(pcy)
+        fx, fy = P[0] * (ci - pcx) + P[1] * (cj - pcy), P[2] * (ci - pcx) + P[3] * (cj - pcy)
     nx, ny = window.nx, window.ny
     cp.nx, cp.ny = nx, ny
-    cp.ci, cp.cj = int(cij[0]), int(cij[1])
-    for k in range(4):
-        cp.P[k] = float(P.reshape(-1)[k])
-    cp.frac[0], cp.frac[1] = float(frac[0]), float(frac[1])
-    cp.pixel_area = float(window.pixel_area)
+    cp.ci, cp.cj = int(ci), int(cj)
+    cp.P[:] = P
+    cp.frac[0], cp.frac[1] = fx, fy
+    cp.pixel_area = window._area
     cp.softening = float(model.softening)
     cp.tolerance = float(model.sampling_tolerance)
     if kind == "flat_sky":
@@ -194,56 +229,65 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
         cp.dvdr[s] = float(v)
 
     dev = AP_config.ap_device
-    planes = torch.empty((cp.nchan, ny, nx), dtype=torch.float32, device=dev)
+    npl = cp.nchan * ny * nx
+    planes = _buffer(model, "planes", 4 * npl, dev)
     wsb = lib.apx_render_workspace_bytes(nx * ny)
-    ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+    ws = _buffer(model, "render_ws", wsb, dev)
     st = L.stream()
-    L.check(lib.apx_render_component(ctypes.byref(cp), L.ptr(planes), L.ptr(ws), wsb, st))
+    _launch("render", nx * ny, lambda: L.check(
+        lib.apx_render_component(ctypes.byref(cp), L.ptr(planes), L.ptr(ws), wsb, st)))
 
     # shifted PSF (or unit kernel)
     if psf_full:
-        pdata = model.psf.data.to(torch.float32).contiguous()
+        pdata = model.psf.data
+        if pdata.dtype != torch.float32 or not pdata.is_contiguous():
+            pdata = pdata.to(torch.float32).contiguous()
         ph, pw = int(pdata.shape[0]), int(pdata.shape[1])
         if shift is not None:
-            kernels = torch.empty((3, ph + 2, pw + 2), dtype=torch.float32, device=dev)
-            pinv = (ctypes.c_double * 4)(*[float(v) for v in window.pixelscale_inv.reshape(-1)])
-            L.check(lib.apx_psf_shift(L.ptr(pdata), ph, pw, float(shift[0]), float(shift[1]), pinv,
+            kbuf = _buffer(model, "kernels", 4 * 3 * (ph + 2) * (pw + 2), dev)
+            pinv = (ctypes.c_double * 4)(*window._Pi)
+            L.check(lib.apx_psf_shift(L.ptr(pdata), ph, pw, shift[0], shift[1], pinv,
                                       float(dvdr.get(L.APX_P_CX, 1.0)), float(dvdr.get(L.APX_P_CY, 1.0)),
-                                      L.ptr(kernels), st))
+                                      L.ptr(kbuf), st))
+            kptr, kh, kw = L.ptr(kbuf), ph + 2, pw + 2
         else:
             kernels = torch.flip(pdata / pdata.sum(), dims=(0, 1)).reshape(1, ph, pw).contiguous()
-        kh, kw = int(kernels.shape[1]), int(kernels.shape[2])
+            model.__dict__.setdefault("_apx_buffers", {})["kernels_t"] = kernels
+            kptr, kh, kw = L.ptr(kernels), ph, pw
     else:
-        kernels = _unit_kernel(dev)
-        kh = kw = 1
+        kptr, kh, kw = L.ptr(_unit_kernel(dev)), 1, 1
 
     # crop back to the sampling window, then window-aware accumulation
     sample_window = req.window
+    cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
+    cws = _buffer(model, "conv_ws", cwsb, dev)
     job = L.ConvJob()
     job.nx, job.ny, job.nchan, job.kh, job.kw = nx, ny, cp.nchan, kh, kw
-    for c in range(L.APX_MAX_CHAN):
-        job.chan_col[c] = -1
+    job.chan_col[:] = [-1] * L.APX_MAX_CHAN
     job.dk_col[0] = job.dk_col[1] = -1
     if req.image is not None:
-        (dy, dx), (sy, sx), (h, w) = _box(req.image.window, sample_window)
-        j1 = _job_copy(job)
-        j1.nchan = 1
-        j1.x0, j1.y0, j1.ow, j1.oh = bx + sx, by + sy, w, h
+        region = req.region if req.region is not None else req.image.window
+        (dy, dx), (sy, sx), (h, w) = _box(req.image.window, sample_window, region)
         img = req.image.data
         if img.dtype != torch.float32 or not img.is_contiguous():
             raise SpecificationConflict("model image must be contiguous float32")
+        j1 = _job_copy(job)
+        j1.nchan = 1
+        j1.x0, j1.y0, j1.ow, j1.oh = bx + sx, by + sy, w, h
         j1.img, j1.img_row, j1.img_x0, j1.img_y0 = L.ptr(img), img.shape[1], dx, dy
         j1.img_w, j1.img_h = img.shape[1], img.shape[0]
         if h > 0 and w > 0:
-            L.check(lib.apx_convolve(ctypes.byref(j1), L.ptr(planes), L.ptr(kernels), st))
+            _launch("convolve", 2.0 * kh * kw * h * w, lambda: L.check(
+                lib.apx_convolve(ctypes.byref(j1), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
     if req.jac is not None and slot_cols:
-        (dy, dx), (sy, sx), (h, w) = _box(req.jac.window, sample_window)
-        j2 = _job_copy(job)
-        j2.x0, j2.y0, j2.ow, j2.oh = bx + sx, by + sy, w, h
+        region = req.region if req.region is not None else req.jac.window
+        (dy, dx), (sy, sx), (h, w) = _box(req.jac.window, sample_window, region)
         jd = req.jac.data
-        npj = int(jd.shape[2])
         if not jd.is_contiguous() or jd.dtype != torch.float32:
             raise SpecificationConflict("Jacobian must be contiguous float32")
+        npj = int(jd.shape[2])
+        j2 = _job_copy(job)
+        j2.x0, j2.y0, j2.ow, j2.oh = bx + sx, by + sy, w, h
         j2.jac, j2.jac_row, j2.jac_np, j2.jac_x0, j2.jac_y0 = L.ptr(jd), jd.shape[1] * npj, npj, dx, dy
         j2.jac_w, j2.jac_h = jd.shape[1], jd.shape[0]
         for c, s in enumerate(chan_slots):
@@ -252,7 +296,9 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
             j2.dk_col[0] = slot_cols.get(L.APX_P_CX, -1)
             j2.dk_col[1] = slot_cols.get(L.APX_P_CY, -1)
         if h > 0 and w > 0:
-            L.check(lib.apx_convolve(ctypes.byref(j2), L.ptr(planes), L.ptr(kernels), st))
+            nk = (cp.nchan - 1) + (2 if j2.dk_col[0] >= 0 or j2.dk_col[1] >= 0 else 0)
+            _launch("convolve_jac", 2.0 * kh * kw * h * w * nk, lambda: L.check(
+                lib.apx_convolve(ctypes.byref(j2), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
     return ws
 
 

@@ -52,9 +52,14 @@ int apx_psf_shift(const float* psf, int32_t ph, int32_t pw, double shift_x, doub
                         static_cast<hipStream_t>(stream));
 }
 
+int64_t apx_convolve_workspace_bytes(int32_t kh, int32_t kw) {
+  return apx::convolve_workspace_bytes(kh, kw);
+}
+
 int apx_convolve(const apx_conv_job* job, const float* planes, const float* kernels,
-                 void* stream) {
-  return apx::convolve(job, planes, kernels, static_cast<hipStream_t>(stream));
+                 void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::convolve(job, planes, kernels, workspace, workspace_bytes,
+                       static_cast<hipStream_t>(stream));
 }
 
 int64_t apx_reduce_workspace_bytes(int64_t npix, int32_t np) {

@@ -22,8 +22,9 @@ int render_selected_count(const void* workspace, hipStream_t stream, int32_t* co
 
 int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
               double dvdr_x, double dvdr_y, float* kernels, hipStream_t stream);
-int convolve(const apx_conv_job* job, const float* planes, const float* kernels,
-             hipStream_t stream);
+int64_t convolve_workspace_bytes(int32_t kh, int32_t kw);
+int convolve(const apx_conv_job* job, const float* planes, const float* kernels, void* workspace,
+             int64_t ws_bytes, hipStream_t stream);
 
 int64_t reduce_workspace_bytes(int64_t npix, int32_t np);
 int chi2(const float* model, const float* data, const float* weight, const uint8_t* mask,

@@ -105,7 +105,9 @@ __global__ __launch_bounds__(256) void convolve_kernel(apx_conv_job job, const f
   const int64_t npl = (int64_t)job.nx * job.ny;
   const int knum = kh * kw;
   const int ox = ox0 + lx;
-  for (int c = 0; c < job.nchan; ++c) {
+  // plane 0 is needed for the value and the centre derivatives only
+  const int c_first = (job.img || DK) ? 0 : 1;
+  for (int c = c_first; c < job.nchan; ++c) {
     const bool centre = DK && c == 0;
     float acc[kRY], accx[kRY], accy[kRY];
 #pragma unroll
@@ -169,6 +171,116 @@ __global__ __launch_bounds__(256) void convolve_kernel(apx_conv_job job, const f
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Fast path (kernel side <= kFastMax): one workgroup = 64 columns x 8 rows of
+// output, four wavefronts.  Each wavefront owns interleaved blocks of 8 input
+// rows of the staged tile and all 8 output rows: for a tap column b it reads 8
+// inputs from LDS and 16 taps through the scalar cache (one burst), then does
+// 64 FMAs.  Taps come pre-arranged as KT[k][b][u] = K[u - 7][b] (zero outside
+// the kernel) so every (input row, output row) pair finds its tap at
+// u = t - r + 7 with no bounds tests.  The four partial sums are combined in
+// LDS in a fixed order (deterministic).
+constexpr int kRYF = 8;      // output rows per workgroup (one per accumulator)
+constexpr int kFastMax = 89; // largest kernel side whose tile fits 64 KiB of LDS
+
+__host__ __device__ inline int fast_nt(int kh) { return (kh + kRYF - 1 + 7) / 8; }
+__host__ __device__ inline int fast_kts(int kh) { return 8 * fast_nt(kh) + 8; }
+
+__global__ __launch_bounds__(256) void conv_prep_kernel(const float* __restrict__ kernels, int nk,
+                                                        int kh, int kw, int kts, float* kt) {
+  const int n = nk * kw * kts;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+    const int u = e % kts, b = (e / kts) % kw, k = e / (kts * kw);
+    const int a = u - (kRYF - 1);
+    kt[e] = (a >= 0 && a < kh) ? kernels[(k * kh + a) * kw + b] : 0.0f;
+  }
+}
+
+template <int NK0>
+__global__ __launch_bounds__(256) void convolve_fast_kernel(apx_conv_job job,
+                                                            const float* __restrict__ planes,
+                                                            const float* __restrict__ kt) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ox0 = blockIdx.x * 64, oy0 = blockIdx.y * kRYF;
+  const int kh = job.kh, kw = job.kw, ch = (kh - 1) / 2, cw = (kw - 1) / 2;
+  const int nT = fast_nt(kh), kts = fast_kts(kh);
+  const int rows = 8 * nT, LW = 64 + kw - 1;
+  const int gx0 = job.x0 + ox0 - cw, gy0 = job.y0 + oy0 - ch;
+  const int64_t npl = (int64_t)job.nx * job.ny;
+  const int c_first = (job.img || NK0 > 1) ? 0 : 1;
+  for (int c = c_first; c < job.nchan; ++c) {
+    const float* plane = planes + c * npl;
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * LW; e += 256) {
+      const int ty = e / LW, tx = e % LW;
+      const int gy = gy0 + ty, gx = gx0 + tx;
+      smem[e] = (gy >= 0 && gy < job.ny && gx >= 0 && gx < job.nx) ? plane[(int64_t)gy * job.nx + gx]
+                                                                  : 0.0f;
+    }
+    __syncthreads();
+    constexpr int NKmax = NK0;
+    const int nk = (c == 0) ? NK0 : 1;
+    float acc[NKmax][kRYF];
+#pragma unroll
+    for (int k = 0; k < NKmax; ++k)
+#pragma unroll
+      for (int r = 0; r < kRYF; ++r) acc[k][r] = 0.0f;
+    // wave index made provably uniform so tap addresses go to the scalar unit
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    for (int j = wu; j < nT; j += 4) {
+      const int t0 = 8 * j;
+      const float* trow = smem + t0 * LW + lane;
+      const float* ktj = kt + t0;
+      for (int b = 0; b < kw; ++b) {
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = trow[i * LW + b];
+#pragma unroll
+        for (int k = 0; k < NKmax; ++k) {
+          if (k < nk) {
+            const float4* kp = reinterpret_cast<const float4*>(ktj + ((int64_t)k * kw + b) * kts);
+            const float4 q0 = kp[0], q1 = kp[1], q2 = kp[2], q3 = kp[3];
+            const float kk[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                  q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+              for (int r = 0; r < kRYF; ++r) acc[k][r] = fmaf(kk[i - r + 7], v[i], acc[k][r]);
+          }
+        }
+      }
+    }
+    // combine the four wavefronts' partial sums: red[w][k][r][lane]
+    __syncthreads();
+    float* red = smem;
+#pragma unroll
+    for (int k = 0; k < NKmax; ++k)
+      if (k < nk)
+#pragma unroll
+        for (int r = 0; r < kRYF; ++r) red[((w * NKmax + k) * kRYF + r) * 64 + lane] = acc[k][r];
+    __syncthreads();
+    const int ox = ox0 + lane;
+    for (int e = w; e < nk * kRYF; e += 4) {  // (kernel k, row r) per wavefront
+      const int k = e / kRYF, r = e % kRYF;
+      float s = 0.0f;
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) s += red[((ww * NKmax + k) * kRYF + r) * 64 + lane];
+      const int oy = oy0 + r;
+      if (ox >= job.ow || oy >= job.oh) continue;
+      if (c == 0 && k == 0) {
+        if (job.img) job.img[(int64_t)(job.img_y0 + oy) * job.img_row + (job.img_x0 + ox)] += s;
+      } else if (job.jac) {
+        float* d = job.jac + (int64_t)(job.jac_y0 + oy) * job.jac_row +
+                   (int64_t)(job.jac_x0 + ox) * job.jac_np;
+        const int col = (c == 0) ? job.dk_col[k - 1] : job.chan_col[c];
+        if (col >= 0) d[col] += s;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
@@ -183,8 +295,13 @@ int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, co
   return check_launch("psf_shift");
 }
 
-int convolve(const apx_conv_job* jp, const float* planes, const float* kernels,
-             hipStream_t stream) {
+int64_t convolve_workspace_bytes(int32_t kh, int32_t kw) {
+  if (kh < 1 || kw < 1) return 0;
+  return (int64_t)sizeof(float) * 3 * kw * fast_kts(kh) + 256;
+}
+
+int convolve(const apx_conv_job* jp, const float* planes, const float* kernels, void* workspace,
+             int64_t ws_bytes, hipStream_t stream) {
   if (!jp || !planes || !kernels) return set_error(APX_E_INVALID, "convolve: null pointer");
   const apx_conv_job job = *jp;
   if (job.kh < 1 || job.kw < 1 || (job.kh % 2) == 0 || (job.kw % 2) == 0)
@@ -219,6 +336,24 @@ int convolve(const apx_conv_job* jp, const float* planes, const float* kernels,
     for (int k = 0; k < 2; ++k)
       if (job.dk_col[k] >= job.jac_np)
         return set_error(APX_E_INVALID, "convolve: centre column out of range");
+  }
+  const int nk0 = dk ? 3 : 1;
+  if (job.kh <= kFastMax && job.kw <= kFastMax) {
+    if (!workspace || ws_bytes < convolve_workspace_bytes(job.kh, job.kw))
+      return set_error(APX_E_INVALID, "convolve: workspace too small");
+    float* kt = static_cast<float*>(workspace);
+    const int kts = fast_kts(job.kh);
+    const int n = nk0 * job.kw * kts;
+    hipLaunchKernelGGL(conv_prep_kernel, dim3((unsigned)std::min(64, (n + 255) / 256)), dim3(256), 0,
+                       stream, kernels, nk0, job.kh, job.kw, kts, kt);
+    const size_t tile = (size_t)8 * fast_nt(job.kh) * (size_t)(64 + job.kw - 1);
+    const size_t lds = sizeof(float) * std::max(tile, (size_t)4 * nk0 * kRYF * 64);
+    const dim3 grid((unsigned)((job.ow + 63) / 64), (unsigned)((job.oh + kRYF - 1) / kRYF));
+    if (dk)
+      hipLaunchKernelGGL(convolve_fast_kernel<3>, grid, dim3(256), lds, stream, job, planes, kt);
+    else
+      hipLaunchKernelGGL(convolve_fast_kernel<1>, grid, dim3(256), lds, stream, job, planes, kt);
+    return check_launch("convolve");
   }
   const int KR = std::min(job.kh, kKR);
   const size_t lds = sizeof(float) * (size_t)(kTY + KR - 1) * (size_t)(kTX + job.kw - 1);

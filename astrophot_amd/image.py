@@ -35,10 +35,29 @@ def _dev(v, dtype=None):
     return torch.as_tensor(np.asarray(v), dtype=dtype, device=AP_config.ap_device)
 
 
+def _pair(v):
+    """two host floats from a number pair (tensor, ndarray, sequence)"""
+    if isinstance(v, torch.Tensor):
+        v = v.detach().cpu().reshape(-1).tolist()
+    elif isinstance(v, np.ndarray):
+        v = v.reshape(-1).tolist()
+    return (float(v[0]), float(v[1]))
+
+
+def _round_half_even(x):
+    """torch.round / numpy round semantics (ties to even), on a float."""
+    return float(round(x))
+
+
 class Window:
     """A window on the tangent plane: ``pixel_shape`` (nx, ny) pixels of
     parallelogram ``pixelscale``; pixel ``reference_imageij`` is pinned to plane
-    point ``reference_imagexy`` (image/wcs.py:561-584)."""
+    point ``reference_imagexy`` (image/wcs.py:561-584).
+
+    The geometry is kept in Python floats (IEEE double, the arithmetic the
+    reference does in float64 torch) so the per-call host cost stays small; the
+    properties return float64 tensors as the reference's do.
+    """
 
     def __init__(self, *, pixel_shape=None, origin=None, center=None, pixelscale=None,
                  reference_imageij=(-0.5, -0.5), reference_imagexy=(0.0, 0.0), state=None, **kwargs):
@@ -50,70 +69,92 @@ class Window:
         if origin is not None and center is not None:
             raise SpecificationConflict("Please provide only one reference position for the window")
         if origin is not None:  # window_object.py:143-150
-            self.reference_imageij = (-0.5, -0.5)
-            self.reference_imagexy = origin
+            self._rij = (-0.5, -0.5)
+            self._rxy = _pair(origin)
         elif center is not None:  # window_object.py:151-160
-            self.reference_imageij = self.pixel_shape.to(F64) / 2 - 0.5
-            self.reference_imagexy = center
+            self._rij = (self._nx / 2 - 0.5, self._ny / 2 - 0.5)
+            self._rxy = _pair(center)
         else:
-            self.reference_imageij = reference_imageij
-            self.reference_imagexy = reference_imagexy
+            self._rij = _pair(reference_imageij)
+            self._rxy = _pair(reference_imagexy)
 
     # -- PPCS (wcs.py:492-639) ----------------------------------------------
+    def _set_P(self, P):
+        self._P = P
+        a, b, c, d = P
+        det = a * d - b * c
+        self._area = abs(det)
+        self._Pi = (d / det, -b / det, -c / det, a / det)
+
     @property
     def pixelscale(self):
-        return self._pixelscale
+        return torch.tensor(self._P, dtype=F64).reshape(2, 2)
 
     @pixelscale.setter
     def pixelscale(self, pix):
-        p = _h(pix).clone()
+        p = _h(pix).reshape(-1)
         if p.numel() == 1:
-            p = torch.eye(2, dtype=F64) * p.reshape(())
-        self._pixelscale = p.reshape(2, 2)
-        self._pixel_area = torch.linalg.det(self._pixelscale).abs()
-        self._pixelscale_inv = torch.linalg.inv(self._pixelscale)
+            v = float(p[0])
+            self._set_P((v, 0.0, 0.0, v))
+        else:
+            self._set_P(tuple(float(x) for x in p.tolist()))
 
     @property
     def pixel_area(self):
-        return self._pixel_area
+        return torch.tensor(self._area, dtype=F64)
 
     @property
     def pixel_length(self):
-        return self._pixel_area.sqrt()
+        return torch.tensor(math.sqrt(self._area), dtype=F64)
 
     @property
     def pixelscale_inv(self):
-        return self._pixelscale_inv
+        return torch.tensor(self._Pi, dtype=F64).reshape(2, 2)
 
     @property
     def reference_imageij(self):
-        return self._reference_imageij
+        return torch.tensor(self._rij, dtype=F64)
 
     @reference_imageij.setter
     def reference_imageij(self, v):
-        self._reference_imageij = _h(v).clone()
+        self._rij = _pair(v)
 
     @property
     def reference_imagexy(self):
-        return self._reference_imagexy
+        return torch.tensor(self._rxy, dtype=F64)
 
     @reference_imagexy.setter
     def reference_imagexy(self, v):
-        self._reference_imagexy = _h(v).clone()
+        self._rxy = _pair(v)
 
     @property
     def pixel_shape(self):
-        return self._pixel_shape
+        return torch.tensor([self._nx, self._ny], dtype=torch.int32)
 
     @pixel_shape.setter
     def pixel_shape(self, shape):
-        self._pixel_shape = None if shape is None else torch.round(_h(shape)).to(torch.int32)
+        if shape is None:
+            self._nx = self._ny = None
+            return
+        x, y = _pair(shape)
+        self._nx, self._ny = int(_round_half_even(x)), int(_round_half_even(y))
+
+    # float-level transforms
+    def _p2x(self, i, j):
+        P, r, o = self._P, self._rij, self._rxy
+        di, dj = i - r[0], j - r[1]
+        return (P[0] * di + P[1] * dj + o[0], P[2] * di + P[3] * dj + o[1])
+
+    def _x2p(self, x, y):
+        Pi, r, o = self._Pi, self._rij, self._rxy
+        dx, dy = x - o[0], y - o[1]
+        return (Pi[0] * dx + Pi[1] * dy + r[0], Pi[2] * dx + Pi[3] * dy + r[1])
 
     def pixel_to_plane(self, ij):
-        return self.pixelscale @ (_h(ij) - self.reference_imageij) + self.reference_imagexy
+        return torch.tensor(self._p2x(*_pair(ij)), dtype=F64)
 
     def plane_to_pixel(self, xy):
-        return self.pixelscale_inv @ (_h(xy) - self.reference_imagexy) + self.reference_imageij
+        return torch.tensor(self._x2p(*_pair(xy)), dtype=F64)
 
     def pixel_to_plane_delta(self, d):
         return self.pixelscale @ _h(d)
@@ -124,122 +165,125 @@ class Window:
     # -- window properties (window_object.py:165-239) ---------------------
     @property
     def nx(self):
-        return int(self.pixel_shape[0])
+        return self._nx
 
     @property
     def ny(self):
-        return int(self.pixel_shape[1])
+        return self._ny
 
     @property
     def size(self):
-        return self.nx * self.ny
+        return self._nx * self._ny
+
+    def _end(self):
+        P = self._P
+        return (P[0] * self._nx + P[1] * self._ny, P[2] * self._nx + P[3] * self._ny)
+
+    def _origin(self):
+        return self._p2x(-0.5, -0.5)
 
     @property
     def end(self):
-        return self.pixel_to_plane_delta(self.pixel_shape.to(F64))
+        return torch.tensor(self._end(), dtype=F64)
 
     @property
     def origin(self):
-        return self.pixel_to_plane(-0.5 * torch.ones(2, dtype=F64))
+        return torch.tensor(self._origin(), dtype=F64)
 
     @property
     def center(self):
-        return self.origin + self.end / 2
+        o, e = self._origin(), self._end()
+        return torch.tensor((o[0] + e[0] / 2, o[1] + e[1] / 2), dtype=F64)
 
     @property
     def shape(self):
-        S1 = self.pixel_shape.to(F64).clone()
-        S1[1] = 0.0
-        S2 = self.pixel_shape.to(F64).clone()
-        S2[0] = 0.0
-        return torch.stack((torch.linalg.norm(self.pixelscale @ S1), torch.linalg.norm(self.pixelscale @ S2)))
+        P = self._P
+        return torch.tensor((math.hypot(P[0] * self._nx, P[2] * self._nx),
+                             math.hypot(P[1] * self._ny, P[3] * self._ny)), dtype=F64)
 
     def copy(self, **kwargs):
         w = Window.__new__(Window)
-        w._pixelscale = self._pixelscale.clone()
-        w._pixel_area = self._pixel_area.clone()
-        w._pixelscale_inv = self._pixelscale_inv.clone()
-        w._pixel_shape = self._pixel_shape.clone()
-        w._reference_imageij = self._reference_imageij.clone()
-        w._reference_imagexy = self._reference_imagexy.clone()
+        w._P, w._Pi, w._area = self._P, self._Pi, self._area
+        w._nx, w._ny = self._nx, self._ny
+        w._rij, w._rxy = self._rij, self._rxy
         if "pixelscale" in kwargs:
             w.pixelscale = kwargs["pixelscale"]
         if "pixel_shape" in kwargs:
             w.pixel_shape = kwargs["pixel_shape"]
         if "origin" in kwargs:
-            w.reference_imageij = (-0.5, -0.5)
-            w.reference_imagexy = kwargs["origin"]
+            w._rij = (-0.5, -0.5)
+            w._rxy = _pair(kwargs["origin"])
         if "reference_imageij" in kwargs:
-            w.reference_imageij = kwargs["reference_imageij"]
+            w._rij = _pair(kwargs["reference_imageij"])
         if "reference_imagexy" in kwargs:
-            w.reference_imagexy = kwargs["reference_imagexy"]
+            w._rxy = _pair(kwargs["reference_imagexy"])
         return w
 
     def rescale_pixel(self, scale):
         """window_object.py:233-239."""
+        scale = float(scale)
         return self.copy(pixelscale=self.pixelscale * scale,
-                         pixel_shape=self.pixel_shape.to(F64) // scale,
-                         reference_imageij=(self.reference_imageij + 0.5) / scale - 0.5)
+                         pixel_shape=(self._nx // scale, self._ny // scale),
+                         reference_imageij=((self._rij[0] + 0.5) / scale - 0.5, (self._rij[1] + 0.5) / scale - 0.5))
 
     def pixel_shift(self, shift):
         """window_object.py:284-290."""
-        self.reference_imageij = self.reference_imageij - _h(shift)
+        s = _pair(shift)
+        self._rij = (self._rij[0] - s[0], self._rij[1] - s[1])
         return self
 
     def shift(self, shift):
-        self.reference_imagexy = self.reference_imagexy + _h(shift)
+        s = _pair(shift)
+        self._rxy = (self._rxy[0] + s[0], self._rxy[1] + s[1])
         return self
 
-    def _pix(self, pixels):
-        pixels = [int(p) if not isinstance(p, torch.Tensor) else p for p in pixels]
-        return _h([float(p) for p in pixels])
+    @staticmethod
+    def _pix(pixels):
+        p = [float(v) for v in (pixels.tolist() if isinstance(pixels, torch.Tensor) else pixels)]
+        if len(p) == 1:
+            p = p * 2
+        if len(p) not in (2, 4):
+            raise ValueError(f"Unrecognized pixel pad/crop format: {pixels}")
+        return p
 
     def pad_pixel(self, pixels):
         """window_object.py:367-388."""
         p = self._pix(pixels)
-        if len(p) == 1:
-            p = p.repeat(2)
-        if len(p) == 2:
-            self.pixel_shape = self.pixel_shape.to(F64) + 2 * p
-            self.reference_imageij = self.reference_imageij + p
-        elif len(p) == 4:
-            self.pixel_shape = self.pixel_shape.to(F64) + p[:2] + p[2:]
-            self.reference_imageij = self.reference_imageij + p[:2]
-        else:
-            raise ValueError(f"Unrecognized pixel pad format: {pixels}")
+        lo = p[:2]
+        hi = p[:2] if len(p) == 2 else p[2:]
+        self.pixel_shape = (self._nx + lo[0] + hi[0], self._ny + lo[1] + hi[1])
+        self._rij = (self._rij[0] + lo[0], self._rij[1] + lo[1])
         return self
 
     def crop_pixel(self, pixels):
         """window_object.py:335-356."""
         p = self._pix(pixels)
-        if len(p) == 1:
-            p = p.repeat(2)
-        if len(p) == 2:
-            self.pixel_shape = self.pixel_shape.to(F64) - 2 * p
-            self.reference_imageij = self.reference_imageij - p
-        elif len(p) == 4:
-            self.pixel_shape = self.pixel_shape.to(F64) - p[:2] - p[2:]
-            self.reference_imageij = self.reference_imageij - p[:2]
-        else:
-            raise ValueError(f"Unrecognized pixel crop format: {pixels}")
+        lo = p[:2]
+        hi = p[:2] if len(p) == 2 else p[2:]
+        self.pixel_shape = (self._nx - lo[0] - hi[0], self._ny - lo[1] - hi[1])
+        self._rij = (self._rij[0] - lo[0], self._rij[1] - lo[1])
         return self
 
     def crop_to_pixel(self, pixels):
         """window_object.py:358-365; pixels = [[xmin, xmax], [ymin, ymax]]."""
-        p = _h(pixels)
-        self.reference_imageij = self.reference_imageij - p[:, 0]
-        self.pixel_shape = p[:, 1] - p[:, 0]
+        p = [[float(a), float(b)] for a, b in (pixels.tolist() if isinstance(pixels, torch.Tensor) else pixels)]
+        self._rij = (self._rij[0] - p[0][0], self._rij[1] - p[1][0])
+        self.pixel_shape = (p[0][1] - p[0][0], p[1][1] - p[1][0])
         return self
 
     @staticmethod
     def _get_indices(ref_window, obj_window):
         """window_object.py:241-253: (row slice, column slice) of ref's pixels
         covered by obj."""
-        o = torch.round(ref_window.plane_to_pixel(obj_window.origin) + 0.5).int()
-        o = torch.maximum(torch.zeros_like(o), o)
-        e = torch.round(ref_window.plane_to_pixel(obj_window.origin + obj_window.end) + 0.5).int()
-        e = torch.minimum(ref_window.pixel_shape, e)
-        return slice(int(o[1]), int(e[1])), slice(int(o[0]), int(e[0]))
+        ox, oy = obj_window._origin()
+        ex, ey = obj_window._end()
+        a = ref_window._x2p(ox, oy)
+        b = ref_window._x2p(ox + ex, oy + ey)
+        x0 = max(0, int(_round_half_even(a[0] + 0.5)))
+        y0 = max(0, int(_round_half_even(a[1] + 0.5)))
+        x1 = min(ref_window._nx, int(_round_half_even(b[0] + 0.5)))
+        y1 = min(ref_window._ny, int(_round_half_even(b[1] + 0.5)))
+        return slice(y0, y1), slice(x0, x1)
 
     def get_self_indices(self, obj):
         return self._get_indices(self, obj if isinstance(obj, Window) else obj.window)
@@ -247,21 +291,24 @@ class Window:
     def get_other_indices(self, obj):
         return self._get_indices(obj if isinstance(obj, Window) else obj.window, self)
 
+    def _box(self, other, op):
+        ox, oy = other._origin()
+        ex, ey = other._end()
+        o = self._x2p(ox, oy)
+        e = self._x2p(ox + ex, oy + ey)
+        if op == "and":  # window_object.py:511-523
+            lo = (max(-0.5, o[0]), max(-0.5, o[1]))
+            hi = (min(self._nx - 0.5, e[0]), min(self._ny - 0.5, e[1]))
+        else:  # window_object.py:483-495
+            lo = (min(-0.5, o[0]), min(-0.5, o[1]))
+            hi = (max(float(self._nx), e[0]), max(float(self._ny), e[1]))
+        return self.copy(origin=self._p2x(*lo), pixel_shape=(hi[0] - lo[0], hi[1] - lo[1]))
+
     def __and__(self, other):
-        """window_object.py:511-523."""
-        o = self.plane_to_pixel(other.origin)
-        lo = torch.maximum(-0.5 * torch.ones_like(o), o)
-        e = self.plane_to_pixel(other.origin + other.end)
-        hi = torch.minimum(self.pixel_shape.to(F64) - 0.5, e)
-        return self.copy(origin=self.pixel_to_plane(lo), pixel_shape=hi - lo)
+        return self._box(other, "and")
 
     def __or__(self, other):
-        """window_object.py:483-495."""
-        o = self.plane_to_pixel(other.origin)
-        lo = torch.minimum(-0.5 * torch.ones_like(o), o)
-        e = self.plane_to_pixel(other.origin + other.end)
-        hi = torch.maximum(self.pixel_shape.to(F64), e)
-        return self.copy(origin=self.pixel_to_plane(lo), pixel_shape=hi - lo)
+        return self._box(other, "or")
 
     def overlap_frac(self, other):
         overlap = self & other
@@ -270,31 +317,29 @@ class Window:
 
     def get_coordinate_meshgrid(self, device="cpu"):
         """window_object.py:390-405: plane coordinates of every pixel centre (2, ny, nx)."""
-        i = torch.arange(self.nx, dtype=F64)
-        j = torch.arange(self.ny, dtype=F64)
+        i = torch.arange(self._nx, dtype=F64)
+        j = torch.arange(self._ny, dtype=F64)
         mi, mj = torch.meshgrid(i, j, indexing="xy")
         c = self.pixelscale @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.reference_imageij.view(2, 1))
-        return (c + self.reference_imagexy.view(2, 1)).reshape(2, self.ny, self.nx).to(device)
+        return (c + self.reference_imagexy.view(2, 1)).reshape(2, self._ny, self._nx).to(device)
 
     def __eq__(self, other):
-        return (torch.all(self.pixel_shape == other.pixel_shape).item()
-                and torch.all(self.pixelscale == other.pixelscale).item()
-                and torch.all(self.pixel_to_plane(torch.zeros(2)) == other.pixel_to_plane(torch.zeros(2))).item())
+        return (self._nx == other._nx and self._ny == other._ny and self._P == other._P
+                and self._p2x(0.0, 0.0) == other._p2x(0.0, 0.0))
 
     def get_state(self):
-        return {"pixel_shape": self.pixel_shape.tolist(), "pixelscale": self.pixelscale.tolist(),
-                "reference_imageij": self.reference_imageij.tolist(),
-                "reference_imagexy": self.reference_imagexy.tolist()}
+        return {"pixel_shape": [self._nx, self._ny], "pixelscale": self.pixelscale.tolist(),
+                "reference_imageij": list(self._rij), "reference_imagexy": list(self._rxy)}
 
     def set_state(self, state):
         self.pixelscale = state.get("pixelscale", 1.0)
         self.pixel_shape = state["pixel_shape"]
-        self.reference_imageij = state.get("reference_imageij", (-0.5, -0.5))
-        self.reference_imagexy = state.get("reference_imagexy", (0.0, 0.0))
+        self._rij = _pair(state.get("reference_imageij", (-0.5, -0.5)))
+        self._rxy = _pair(state.get("reference_imagexy", (0.0, 0.0)))
 
     def __repr__(self):
-        return (f"window pixel_shape: {self.pixel_shape.tolist()}, origin: {self.origin.tolist()}, "
-                f"pixelscale: {self.pixelscale.tolist()}")
+        return (f"window pixel_shape: {[self._nx, self._ny]}, origin: {list(self._origin())}, "
+                f"pixelscale: {list(self._P)}")
 
 
 class Image:

@@ -424,7 +424,12 @@ class Group_Model(AstroPhot_Model):
         return "none"
 
     def sample(self, image=None, window=None, parameters=None):
-        """group_model_object.py:183-231."""
+        """group_model_object.py:183-231.
+
+        When the group allocates its own image, each component renders
+        straight into it, clipped to the component's window: the reference's
+        intermediate component image (model window & group window, zero plus
+        the component) adds the same float values in the same order."""
         if image is None:
             sample_window = True
             image = self.make_model_image(window=window)
@@ -432,8 +437,15 @@ class Group_Model(AstroPhot_Model):
             sample_window = False
         if window is None:
             window = image.window
+        if sample_window and image.window == window:
+            for model in self.models.values():
+                if isinstance(model, Component_Model) and model.mask is None:
+                    _engine.render(model, model._values(),
+                                   _engine.RenderRequest(window=window, image=image, region=model.window & window))
+                else:
+                    image += model(window=window)
+            return image
         working = image[window].blank_copy()
-        working.__class__ = Model_Image
         for model in self.models.values():
             if sample_window:
                 working += model(window=window)

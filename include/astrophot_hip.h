@@ -140,11 +140,15 @@ typedef struct apx_conv_job {
   int32_t dk_col[2];       /* jac columns of conv(plane 0, dK/dc), -1 = none */
 } apx_conv_job;
 
+/* Workspace (bytes) apx_convolve needs for a kh x kw kernel. */
+int64_t apx_convolve_workspace_bytes(int32_t kh, int32_t kw);
+
 /* out(ox, oy) = sum_ab kernel[a][b] * plane[y0+oy-(kh-1)/2+a][x0+ox-(kw-1)/2+b]
  * (kernels pre-flipped by apx_psf_shift).  `kernels` is 3 x kh x kw when
- * dk_col is used, else 1 x kh x kw. */
+ * dk_col is used, else 1 x kh x kw.  The workspace holds the taps re-arranged
+ * for the scalar cache. */
 int apx_convolve(const apx_conv_job* job, const float* planes, const float* kernels,
-                 void* stream);
+                 void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Workspace (bytes) for apx_chi2 / apx_normal_equations / apx_rpp. */
 int64_t apx_reduce_workspace_bytes(int64_t npix, int32_t np);

@@ -133,13 +133,24 @@ def _box(dst_window, src_window, region):
     return (dr.start, dc.start), (sr.start, sc.start), (max(h, 0), max(w, 0))
 
 
-def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
-    """Render ``model`` with parameter ``values`` (host float64) on
-    ``req.window``; accumulate into ``req.image`` / ``req.jac``."""
-    lib = L.load()
+@dataclass
+class Plan:
+    """Launch descriptor and geometry of one component on one sampling window."""
+
+    cp: object            # L.Component
+    window: object        # padded working window
+    bx: int
+    by: int
+    shift: Optional[Tuple[float, float]]
+    psf_full: bool
+    chan_slots: List[int]
+
+
+def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Plan:
+    """Fill the apx_component of ``model`` for sampling ``window``."""
     kind = model._profile
     psf_full = "full" in model.psf_mode and kind != "flat_sky"
-    window = req.window.copy()
+    window = window.copy()
     cx, cy = (float(v) for v in values["center"].reshape(-1).tolist())
     P = window._P
     cp = L.Component()
@@ -215,56 +226,74 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
                                          float(values["Re"]), float(values["q"])) / (nx * ny)
     else:
         cp.mean_reference = 1
-
-    # Jacobian channels
-    slot_cols = req.slot_cols or {}
-    dvdr = req.dvdr or {}
+    # Jacobian / adjoint channels: the centre acts through the shifted PSF when
+    # psf_mode is "full" with a sub-pixel shift, through X, Y otherwise
+    slot_cols = slot_cols or {}
     chan_slots = [s for s in sorted(slot_cols)
                   if not (psf_full and shift is not None and s in (L.APX_P_CX, L.APX_P_CY))]
-    cp.nchan = 1 + (len(chan_slots) if req.jac is not None else 0)
-    if req.jac is not None:
+    cp.nchan = 1 + (len(chan_slots) if channels else 0)
+    if channels:
         for c, s in enumerate(chan_slots):
             cp.chan_slot[c + 1] = s
-    for s, v in dvdr.items():
+    for s, v in (dvdr or {}).items():
         cp.dvdr[s] = float(v)
+    return Plan(cp, window, bx, by, shift, psf_full, chan_slots)
 
+
+def _kernels(model, pl, dvdr, dev, st):
+    """Shifted, normalised, flipped PSF (+ centre derivatives) or a unit kernel:
+    (device pointer, kh, kw, tensor view of the taps)."""
+    lib = L.load()
+    if pl.psf_full:
+        pdata = model.psf.data
+        if pdata.dtype != torch.float32 or not pdata.is_contiguous():
+            pdata = pdata.to(torch.float32).contiguous()
+        ph, pw = int(pdata.shape[0]), int(pdata.shape[1])
+        if pl.shift is not None:
+            kbuf = _buffer(model, "kernels", 4 * 3 * (ph + 2) * (pw + 2), dev)
+            pinv = (ctypes.c_double * 4)(*pl.window._Pi)
+            L.check(lib.apx_psf_shift(L.ptr(pdata), ph, pw, pl.shift[0], pl.shift[1], pinv,
+                                      float(dvdr.get(L.APX_P_CX, 1.0)), float(dvdr.get(L.APX_P_CY, 1.0)),
+                                      L.ptr(kbuf), st))
+            view = kbuf[:4 * 3 * (ph + 2) * (pw + 2)].view(torch.float32).view(3, ph + 2, pw + 2)
+            return L.ptr(kbuf), ph + 2, pw + 2, view
+        kernels = torch.flip(pdata / pdata.sum(), dims=(0, 1)).reshape(1, ph, pw).contiguous()
+        model.__dict__.setdefault("_apx_buffers", {})["kernels_t"] = kernels
+        return L.ptr(kernels), ph, pw, kernels
+    k = _unit_kernel(dev)
+    return L.ptr(k), 1, 1, k
+
+
+def _new_job(pl, kh, kw):
+    job = L.ConvJob()
+    job.nx, job.ny, job.nchan, job.kh, job.kw = pl.cp.nx, pl.cp.ny, pl.cp.nchan, kh, kw
+    job.chan_col[:] = [-1] * L.APX_MAX_CHAN
+    job.dk_col[0] = job.dk_col[1] = -1
+    return job
+
+
+def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
+    """Render ``model`` with parameter ``values`` (host float64) on
+    ``req.window``; accumulate into ``req.image`` / ``req.jac``."""
+    lib = L.load()
+    dvdr = req.dvdr or {}
+    slot_cols = req.slot_cols or {}
+    pl = plan(model, values, req.window, slot_cols, dvdr, channels=req.jac is not None)
+    cp, nx, ny = pl.cp, pl.cp.nx, pl.cp.ny
     dev = AP_config.ap_device
-    npl = cp.nchan * ny * nx
-    planes = _buffer(model, "planes", 4 * npl, dev)
+    planes = _buffer(model, "planes", 4 * cp.nchan * ny * nx, dev)
     wsb = lib.apx_render_workspace_bytes(nx * ny)
     ws = _buffer(model, "render_ws", wsb, dev)
     st = L.stream()
     _launch("render", nx * ny, lambda: L.check(
         lib.apx_render_component(ctypes.byref(cp), L.ptr(planes), L.ptr(ws), wsb, st)))
-
-    # shifted PSF (or unit kernel)
-    if psf_full:
-        pdata = model.psf.data
-        if pdata.dtype != torch.float32 or not pdata.is_contiguous():
-            pdata = pdata.to(torch.float32).contiguous()
-        ph, pw = int(pdata.shape[0]), int(pdata.shape[1])
-        if shift is not None:
-            kbuf = _buffer(model, "kernels", 4 * 3 * (ph + 2) * (pw + 2), dev)
-            pinv = (ctypes.c_double * 4)(*window._Pi)
-            L.check(lib.apx_psf_shift(L.ptr(pdata), ph, pw, shift[0], shift[1], pinv,
-                                      float(dvdr.get(L.APX_P_CX, 1.0)), float(dvdr.get(L.APX_P_CY, 1.0)),
-                                      L.ptr(kbuf), st))
-            kptr, kh, kw = L.ptr(kbuf), ph + 2, pw + 2
-        else:
-            kernels = torch.flip(pdata / pdata.sum(), dims=(0, 1)).reshape(1, ph, pw).contiguous()
-            model.__dict__.setdefault("_apx_buffers", {})["kernels_t"] = kernels
-            kptr, kh, kw = L.ptr(kernels), ph, pw
-    else:
-        kptr, kh, kw = L.ptr(_unit_kernel(dev)), 1, 1
-
+    kptr, kh, kw, _ = _kernels(model, pl, dvdr, dev, st)
     # crop back to the sampling window, then window-aware accumulation
     sample_window = req.window
     cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
     cws = _buffer(model, "conv_ws", cwsb, dev)
-    job = L.ConvJob()
-    job.nx, job.ny, job.nchan, job.kh, job.kw = nx, ny, cp.nchan, kh, kw
-    job.chan_col[:] = [-1] * L.APX_MAX_CHAN
-    job.dk_col[0] = job.dk_col[1] = -1
+    job = _new_job(pl, kh, kw)
+    bx, by = pl.bx, pl.by
     if req.image is not None:
         region = req.region if req.region is not None else req.image.window
         (dy, dx), (sy, sx), (h, w) = _box(req.image.window, sample_window, region)
@@ -290,9 +319,9 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
         j2.x0, j2.y0, j2.ow, j2.oh = bx + sx, by + sy, w, h
         j2.jac, j2.jac_row, j2.jac_np, j2.jac_x0, j2.jac_y0 = L.ptr(jd), jd.shape[1] * npj, npj, dx, dy
         j2.jac_w, j2.jac_h = jd.shape[1], jd.shape[0]
-        for c, s in enumerate(chan_slots):
+        for c, s in enumerate(pl.chan_slots):
             j2.chan_col[c + 1] = slot_cols[s]
-        if psf_full and shift is not None:
+        if pl.psf_full and pl.shift is not None:
             j2.dk_col[0] = slot_cols.get(L.APX_P_CX, -1)
             j2.dk_col[1] = slot_cols.get(L.APX_P_CY, -1)
         if h > 0 and w > 0:
@@ -300,6 +329,69 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
             _launch("convolve_jac", 2.0 * kh * kw * h * w * nk, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j2), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
     return ws
+
+
+def vjp(model, values, window, G, G_window, slot_cols, dvdr, region=None):
+    """Hand-written adjoint of ``render`` (value image only): returns
+    {column: sum over pixels of G * d image / d rep(column)} for the columns
+    in ``slot_cols``, as float64 device scalars in one tensor indexed like
+    ``sorted(slot_cols.values())``.
+
+    G (float32, G_window's shape) is carried back through the convolution
+    (correlation with the unflipped shifted PSF), then apx_render_vjp
+    recomputes the render and reduces against it; the centre columns of a
+    shifted-PSF model are sum(G * conv(deep, dK/dc))."""
+    lib = L.load()
+    dev = AP_config.ap_device
+    st = L.stream()
+    pl = plan(model, values, window, slot_cols, dvdr, channels=True)
+    cp, nx, ny, bx, by = pl.cp, pl.cp.nx, pl.cp.ny, pl.bx, pl.by
+    region = G_window if region is None else region
+    (dy, dx), (sy, sx), (h, w) = _box(G_window, window, region)
+    out = torch.zeros(len(slot_cols), dtype=torch.float64, device=dev)
+    col_index = {c: k for k, c in enumerate(sorted(slot_cols.values()))}
+    if h <= 0 or w <= 0:
+        return out, col_index
+    Gw = torch.zeros((ny, nx), dtype=torch.float32, device=dev)
+    Gw[by + sy:by + sy + h, bx + sx:bx + sx + w] = G[dy:dy + h, dx:dx + w]
+    kptr, kh, kw, kview = _kernels(model, pl, dvdr, dev, st)
+    cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
+    cws = _buffer(model, "conv_ws", cwsb, dev)
+    if pl.psf_full:
+        # B = adjoint of the convolution: correlation with the unflipped taps
+        taps = torch.flip(kview[0], dims=(0, 1)).contiguous()
+        B = torch.zeros((ny, nx), dtype=torch.float32, device=dev)
+        jb = _new_job(pl, kh, kw)
+        jb.nchan = 1
+        jb.x0, jb.y0, jb.ow, jb.oh = 0, 0, nx, ny
+        jb.img, jb.img_row, jb.img_w, jb.img_h, jb.img_x0, jb.img_y0 = L.ptr(B), nx, nx, ny, 0, 0
+        jb.zero_outside = 1
+        L.check(lib.apx_convolve(ctypes.byref(jb), L.ptr(Gw), L.ptr(taps), L.ptr(cws), cwsb, st))
+    else:
+        B = Gw
+    value = _buffer(model, "planes", 4 * ny * nx, dev)
+    wsb = lib.apx_render_workspace_bytes(nx * ny)
+    ws = _buffer(model, "render_ws", wsb, dev)
+    if cp.nchan > 1:
+        g = torch.empty(cp.nchan - 1, dtype=torch.float64, device=dev)
+        L.check(lib.apx_render_vjp(ctypes.byref(cp), L.ptr(B), L.ptr(value), L.ptr(g), L.ptr(ws), wsb, st))
+        for c, s in enumerate(pl.chan_slots):
+            out[col_index[slot_cols[s]]] += g[c]
+    elif pl.psf_full and pl.shift is not None:
+        L.check(lib.apx_render_component(ctypes.byref(cp), L.ptr(value), L.ptr(ws), wsb, st))
+    if pl.psf_full and pl.shift is not None and (L.APX_P_CX in slot_cols or L.APX_P_CY in slot_cols):
+        C = torch.zeros((h, w, 2), dtype=torch.float32, device=dev)
+        jc = _new_job(pl, kh, kw)
+        jc.nchan = 1
+        jc.x0, jc.y0, jc.ow, jc.oh = bx + sx, by + sy, w, h
+        jc.jac, jc.jac_row, jc.jac_w, jc.jac_h, jc.jac_np, jc.jac_x0, jc.jac_y0 = L.ptr(C), 2 * w, w, h, 2, 0, 0
+        jc.dk_col[0], jc.dk_col[1] = 0, 1
+        L.check(lib.apx_convolve(ctypes.byref(jc), L.ptr(value), kptr, L.ptr(cws), cwsb, st))
+        Gb = G[dy:dy + h, dx:dx + w].double()
+        for k, s in enumerate((L.APX_P_CX, L.APX_P_CY)):
+            if s in slot_cols:
+                out[col_index[slot_cols[s]]] += (Gb * C[..., k].double()).sum()
+    return out, col_index
 
 
 def _job_copy(job):

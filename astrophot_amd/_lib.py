@@ -55,7 +55,7 @@ class ConvJob(ctypes.Structure):
         ("img", _p), ("img_row", _i64), ("img_w", _i32), ("img_h", _i32), ("img_x0", _i32), ("img_y0", _i32),
         ("jac", _p), ("jac_row", _i64), ("jac_w", _i32), ("jac_h", _i32), ("jac_np", _i32),
         ("jac_x0", _i32), ("jac_y0", _i32),
-        ("chan_col", _i32 * APX_MAX_CHAN), ("dk_col", _i32 * 2),
+        ("chan_col", _i32 * APX_MAX_CHAN), ("dk_col", _i32 * 2), ("zero_outside", _i32),
     ]
 
 
@@ -64,6 +64,7 @@ SIGNATURES = {
     "apx_last_error": (ctypes.c_char_p, []),
     "apx_render_workspace_bytes": (_i64, [_i64]),
     "apx_render_component": (ctypes.c_int, [ctypes.POINTER(Component), _p, _p, _i64, _p]),
+    "apx_render_vjp": (ctypes.c_int, [ctypes.POINTER(Component), _p, _p, _p, _p, _i64, _p]),
     "apx_render_selected_count": (ctypes.c_int, [_p, _p, ctypes.POINTER(_i32)]),
     "apx_psf_shift": (ctypes.c_int, [_p, _i32, _i32, _f64, _f64, _p, _f64, _f64, _p, _p]),
     "apx_convolve_workspace_bytes": (_i64, [_i32, _i32]),

@@ -1,0 +1,229 @@
+"""Batched independent fits: many cutouts, one Sersic each, fitted together on
+one GPU by the device-resident Levenberg-Marquardt of ``csrc/batch.hip``.
+
+Each cutout is what a reference user would fit with
+
+    tgt = ap.image.Target_Image(data=d, pixelscale=ps, variance=v, psf=psf, origin=o)
+    m = ap.models.AstroPhot_Model(model_type="sersic galaxy model", target=tgt,
+                                  psf_mode="full", parameters=p0)
+    ap.fit.LM(m).fit()
+
+and the batch gives every cutout that result (same render, same LM control
+flow, fit/lm.py:248-493), all cutouts at once: one tick is one LM sub-step of
+every unfinished fit; a tick is captured in a HIP graph and replayed.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import AP_config
+from . import _lib as L
+from .errors import SpecificationConflict
+
+NP = 7
+PARAM_NAMES = ("center_x", "center_y", "q", "PA", "n", "Re", "Ie")
+STATUS = {0: "running", 1: "success", 2: "success by immobility. Convergence not guaranteed",
+          3: "fail. Could not find step to improve Chi^2", 4: "fail. Maximum iterations"}
+
+
+class Galaxy(ctypes.Structure):
+    """apx_batch_galaxy."""
+
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ph", ctypes.c_int32), ("pw", ctypes.c_int32),
+                ("pix_off", ctypes.c_int64), ("psf_off", ctypes.c_int64), ("work_off", ctypes.c_int64),
+                ("tap_off", ctypes.c_int64), ("chunk_off", ctypes.c_int64), ("tile_off", ctypes.c_int64),
+                ("origin", ctypes.c_double * 2)]
+
+
+class Batch(ctypes.Structure):
+    """apx_batch."""
+
+    _p = ctypes.c_void_p
+    _fields_ = [("ngal", ctypes.c_int32), ("max_kernel", ctypes.c_int32),
+                ("npix", ctypes.c_int64), ("nwork", ctypes.c_int64), ("nchunk", ctypes.c_int64),
+                ("ntile", ctypes.c_int64), ("ntap", ctypes.c_int64),
+                ("P", ctypes.c_double * 4), ("softening", ctypes.c_double), ("tolerance", ctypes.c_double),
+                ("quad_level", ctypes.c_int32), ("gridding", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+                ("max_iter", ctypes.c_int32), ("relative_tolerance", ctypes.c_double), ("L0", ctypes.c_double),
+                ("gal", _p), ("data", _p), ("weight", _p), ("psf", _p), ("model0", _p), ("model1", _p),
+                ("jac", _p), ("planes", _p), ("taps", _p), ("state", _p), ("scratch", _p)]
+
+
+SIGNATURES = {
+    "apx_batch_plan": (ctypes.c_int, [ctypes.POINTER(Galaxy), ctypes.c_int32, ctypes.POINTER(Batch)]),
+    "apx_batch_state_bytes": (ctypes.c_int64, []),
+    "apx_batch_scratch_bytes": (ctypes.c_int64, [ctypes.POINTER(Batch)]),
+    "apx_batch_render": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apx_batch_start": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
+    "apx_batch_tick": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p]),
+    "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 6),
+}
+L.SIGNATURES.update(SIGNATURES)
+
+
+@dataclass
+class Cutout:
+    """One fit: data and variance (ny, nx), PSF (odd, unit sum) and the plane
+    coordinates of the cutout's corner (pixel -0.5, -0.5)."""
+
+    data: np.ndarray
+    variance: np.ndarray
+    psf: np.ndarray
+    origin: Sequence[float] = (0.0, 0.0)
+
+
+def values_to_rep(v: np.ndarray) -> np.ndarray:
+    """[cx, cy, q, PA, n, Re, Ie] values -> representation
+    (utils/conversions/optimization.py:6-14, 50-54)."""
+    v = np.asarray(v, dtype=np.float64)
+    r = v.copy()
+    r[..., 2] = np.tan((v[..., 2] - 0.0) * np.pi / 1.0 - np.pi / 2)
+    r[..., 3] = np.mod(v[..., 3], np.pi)
+    r[..., 4] = np.tan((v[..., 4] - 0.36) * np.pi / (8.0 - 0.36) - np.pi / 2)
+    r[..., 5] = v[..., 5] - 1.0 / (v[..., 5] - 0.0)
+    return r
+
+
+def rep_to_values(r: np.ndarray) -> np.ndarray:
+    """Inverse of ``values_to_rep`` (optimization.py:17-25, 50-54)."""
+    r = np.asarray(r, dtype=np.float64)
+    v = r.copy()
+    v[..., 2] = (np.arctan(r[..., 2]) + np.pi / 2) * 1.0 / np.pi
+    v[..., 3] = np.mod(r[..., 3], np.pi)
+    v[..., 4] = (np.arctan(r[..., 4]) + np.pi / 2) * (8.0 - 0.36) / np.pi + 0.36
+    v[..., 5] = (r[..., 5] + np.sqrt(r[..., 5] ** 2 + 4)) * 0.5
+    return v
+
+
+class BatchFit:
+    """All cutouts resident on the GPU; ``fit(rep0)`` runs every LM to the end."""
+
+    def __init__(self, cutouts: List[Cutout], pixelscale=1.0, max_iter=100, relative_tolerance=1e-5, L0=1.0,
+                 sampling_tolerance=1e-2, softening=1e-3, integrate_quad_level=3, integrate_gridding=5,
+                 integrate_max_depth=3):
+        self.lib = L.load()
+        dev = AP_config.ap_device
+        ng = len(cutouts)
+        if ng == 0:
+            raise SpecificationConflict("BatchFit needs at least one cutout")
+        self.cutouts = cutouts
+        gal = (Galaxy * ng)()
+        for k, c in enumerate(cutouts):
+            ny, nx = c.data.shape
+            ph, pw = c.psf.shape
+            gal[k].nx, gal[k].ny, gal[k].ph, gal[k].pw = nx, ny, ph, pw
+            gal[k].origin[0], gal[k].origin[1] = float(c.origin[0]), float(c.origin[1])
+        b = Batch()
+        L.check(self.lib.apx_batch_plan(gal, ng, ctypes.byref(b)))
+        P = torch.as_tensor(pixelscale, dtype=torch.float64).reshape(-1)
+        if P.numel() == 1:
+            P = torch.tensor([P.item(), 0.0, 0.0, P.item()], dtype=torch.float64)
+        b.P[:] = [float(x) for x in P]
+        b.softening, b.tolerance = softening, sampling_tolerance
+        b.quad_level, b.gridding, b.max_depth = integrate_quad_level, integrate_gridding, integrate_max_depth
+        b.max_iter, b.relative_tolerance, b.L0 = max_iter, relative_tolerance, L0
+        self.gal_host = gal
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.gal_dev = torch.frombuffer(bytearray(gal), dtype=torch.uint8).to(dev)
+        self.data = torch.cat([torch.as_tensor(c.data, dtype=torch.float32).reshape(-1) for c in cutouts]).to(dev)
+        self.weight = torch.cat([(1.0 / torch.as_tensor(c.variance, dtype=torch.float64)).to(torch.float32).reshape(-1)
+                                 for c in cutouts]).to(dev)
+        self.psf = torch.cat([torch.as_tensor(c.psf, dtype=torch.float32).reshape(-1) for c in cutouts]).to(dev)
+        self.model0 = torch.zeros(b.npix, **f32)
+        self.model1 = torch.zeros(b.npix, **f32)
+        self.jac = torch.zeros(b.npix * NP, **f32)
+        self.planes = torch.zeros(6 * b.nwork, **f32)
+        self.taps = torch.zeros(b.ntap, **f32)
+        self.state = torch.zeros(ng * self.lib.apx_batch_state_bytes(), dtype=torch.uint8, device=dev)
+        for name in ("gal", "data", "weight", "psf", "model0", "model1", "jac", "planes", "taps", "state"):
+            src = self.gal_dev if name == "gal" else getattr(self, name)
+            setattr(b, name, src.data_ptr())
+        self.scratch = torch.zeros(self.lib.apx_batch_scratch_bytes(ctypes.byref(b)), dtype=torch.uint8, device=dev)
+        b.scratch = self.scratch.data_ptr()
+        self.b = b
+        self.ngal = ng
+        self.npix = b.npix
+        self._rep = torch.zeros((ng, NP), dtype=torch.float64, device=dev)
+        self._chi2 = torch.zeros(ng, dtype=torch.float64, device=dev)
+        self._status = torch.zeros(ng, dtype=torch.int32, device=dev)
+        self._iters = torch.zeros(ng, dtype=torch.int32, device=dev)
+        self._done = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.graph = None
+        self.ticks = 0
+
+    # -- rendering --------------------------------------------------------
+    def render(self, rep) -> torch.Tensor:
+        """All cutouts rendered at ``rep`` (ngal x 7, representation) into one
+        flat float32 tensor (cutouts concatenated row-major)."""
+        rep = torch.as_tensor(rep, dtype=torch.float64, device=self.data.device).contiguous()
+        out = torch.zeros(self.npix, dtype=torch.float32, device=self.data.device)
+        L.check(self.lib.apx_batch_render(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), L.stream()))
+        return out
+
+    def split(self, flat: torch.Tensor) -> List[torch.Tensor]:
+        out = []
+        for g in self.gal_host:
+            out.append(flat[g.pix_off:g.pix_off + g.nx * g.ny].view(g.ny, g.nx))
+        return out
+
+    # -- fitting ------------------------------------------------------------
+    def start(self, rep0):
+        rep0 = torch.as_tensor(rep0, dtype=torch.float64, device=self.data.device).contiguous()
+        self._rep0 = rep0
+        L.check(self.lib.apx_batch_start(ctypes.byref(self.b), rep0.data_ptr(), L.stream()))
+        self.ticks = 0
+
+    def tick(self):
+        L.check(self.lib.apx_batch_tick(ctypes.byref(self.b), L.stream()))
+        self.ticks += 1
+
+    def done_count(self) -> int:
+        L.check(self.lib.apx_batch_results(ctypes.byref(self.b), self._rep.data_ptr(), self._chi2.data_ptr(),
+                                           self._status.data_ptr(), self._iters.data_ptr(), self._done.data_ptr(),
+                                           L.stream()))
+        return int(self._done.item())
+
+    def capture(self, ticks_per_graph: int = 4):
+        """Capture ``ticks_per_graph`` ticks in one HIP graph (after one eager
+        tick has loaded every kernel)."""
+        self.graph = torch.cuda.CUDAGraph()
+        stream = torch.cuda.Stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            with torch.cuda.graph(self.graph, stream=stream):
+                for _ in range(ticks_per_graph):
+                    L.check(self.lib.apx_batch_tick(ctypes.byref(self.b), L.stream()))
+        torch.cuda.current_stream().wait_stream(stream)
+        self.ticks_per_graph = ticks_per_graph
+
+    def fit(self, rep0, max_ticks: int = 4000, check_every: int = 8, use_graph: bool = True):
+        """Run every fit to completion.  Returns a dict of host arrays: rep,
+        values, chi2 (reduced), status, iterations."""
+        self.start(rep0)
+        self.tick()
+        if use_graph:
+            if self.graph is None:
+                self.capture(check_every)
+            while self.ticks < max_ticks:
+                self.graph.replay()
+                self.ticks += self.ticks_per_graph
+                if self.done_count() == self.ngal:
+                    break
+        else:
+            while self.ticks < max_ticks:
+                for _ in range(check_every):
+                    self.tick()
+                if self.done_count() == self.ngal:
+                    break
+        self.done_count()
+        rep = self._rep.cpu().numpy()
+        return {"rep": rep, "values": rep_to_values(rep), "chi2": self._chi2.cpu().numpy(),
+                "status": self._status.cpu().numpy(), "iterations": self._iters.cpu().numpy(),
+                "ticks": self.ticks}

@@ -41,6 +41,12 @@ int apx_render_component(const apx_component* comp, float* planes, void* workspa
                                static_cast<hipStream_t>(stream));
 }
 
+int apx_render_vjp(const apx_component* comp, const float* weights, float* value_plane,
+                   double* grad, void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::render_vjp(comp, weights, value_plane, grad, workspace, workspace_bytes,
+                         static_cast<hipStream_t>(stream));
+}
+
 int apx_render_selected_count(const void* workspace, void* stream, int32_t* count) {
   return apx::render_selected_count(workspace, static_cast<hipStream_t>(stream), count);
 }
@@ -89,6 +95,32 @@ int apx_rpp(const float* J, const float* m0, const float* m1, const float* weigh
 int apx_lm_solve(const double* hess, const double* grad, int32_t np, double L, double* h,
                  int32_t* status, void* stream) {
   return apx::lm_solve(hess, grad, np, L, h, status, static_cast<hipStream_t>(stream));
+}
+
+int apx_batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* batch) {
+  return apx::batch_plan(gal, ngal, batch);
+}
+
+int64_t apx_batch_state_bytes(void) { return apx::batch_state_bytes(); }
+
+int64_t apx_batch_scratch_bytes(const apx_batch* batch) { return apx::batch_scratch_bytes(batch); }
+
+int apx_batch_render(const apx_batch* batch, const double* rep, float* out, void* stream) {
+  return apx::batch_render(batch, rep, out, static_cast<hipStream_t>(stream));
+}
+
+int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream) {
+  return apx::batch_start(batch, rep0, static_cast<hipStream_t>(stream));
+}
+
+int apx_batch_tick(const apx_batch* batch, void* stream) {
+  return apx::batch_tick(batch, static_cast<hipStream_t>(stream));
+}
+
+int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
+                      int32_t* iterations, int32_t* done, void* stream) {
+  return apx::batch_results(batch, rep, chi2, status, iterations, done,
+                            static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

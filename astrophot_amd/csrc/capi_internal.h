@@ -18,6 +18,8 @@ int check_launch(const char* where);
 int64_t render_workspace_bytes(int64_t npix);
 int render_component(const apx_component* cp, float* planes, void* workspace, int64_t ws_bytes,
                      hipStream_t stream);
+int render_vjp(const apx_component* cp, const float* weights, float* value_plane, double* grad,
+               void* workspace, int64_t ws_bytes, hipStream_t stream);
 int render_selected_count(const void* workspace, hipStream_t stream, int32_t* count);
 
 int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
@@ -37,5 +39,14 @@ int rpp(const float* J, const float* m0, const float* m1, const float* weight, c
         int64_t ws_bytes, hipStream_t stream);
 int lm_solve(const double* hess, const double* grad, int32_t np, double L, double* h,
              int32_t* status, hipStream_t stream);
+
+int batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* b);
+int64_t batch_state_bytes();
+int64_t batch_scratch_bytes(const apx_batch* b);
+int batch_render(const apx_batch* b, const double* rep, float* out, hipStream_t stream);
+int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream);
+int batch_tick(const apx_batch* b, hipStream_t stream);
+int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,
+                  int32_t* done, hipStream_t stream);
 
 }  // namespace apx

@@ -1,0 +1,74 @@
+// Direct-convolution pieces shared by the single-component convolution
+// (convolve.hip) and the batched one (batch.hip).
+//
+// Taps are pre-arranged as KT[k][b][u] = K[u - 7][b] (zero outside the
+// kernel), kts floats per tap column, so that the 8 output rows of a
+// wavefront find the tap of (input row t, output row r) at u = t - r + 7.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace apx {
+
+constexpr int kRYF = 8;       // output rows per workgroup (one accumulator each)
+constexpr int kFastMax = 89;  // largest kernel side whose tile fits 64 KiB of LDS
+
+__host__ __device__ inline int fast_nt(int kh) { return (kh + kRYF - 1 + 7) / 8; }
+__host__ __device__ inline int fast_kts(int kh) { return 8 * fast_nt(kh) + 8; }
+
+// Bilinear sample of the 1-px zero-padded PSF at (c - sx, r - sy) with the
+// reference's clamped corners (utils/interpolate.py:282-329), and its
+// derivatives with respect to the shift.
+__device__ __forceinline__ void bilinear(const float* psf, int ph, int pw, int r, int c, double sx,
+                                         double sy, double& val, double& dsx, double& dsy) {
+  const int H2 = ph + 2, W2 = pw + 2;
+  const double x = (double)c - sx, y = (double)r - sy;
+  const int xf = (int)floor(x), yf = (int)floor(y);
+  const int x0 = min(max(xf, 0), W2 - 2), x1 = min(max(xf + 1, 1), W2 - 1);
+  const int y0 = min(max(yf, 0), H2 - 2), y1 = min(max(yf + 1, 1), H2 - 1);
+  auto at = [&](int yy, int xx) -> double {
+    return (yy >= 1 && yy <= ph && xx >= 1 && xx <= pw) ? (double)psf[(yy - 1) * pw + (xx - 1)]
+                                                        : 0.0;
+  };
+  const double fa = at(y0, x0), fb = at(y1, x0), fc = at(y0, x1), fd = at(y1, x1);
+  const double wx1 = (double)x1 - x, wx0 = x - (double)x0;
+  const double wy1 = (double)y1 - y, wy0 = y - (double)y0;
+  val = fa * wx1 * wy1 + fb * wx1 * wy0 + fc * wx0 * wy1 + fd * wx0 * wy0;
+  dsx = -(-fa * wy1 - fb * wy0 + fc * wy1 + fd * wy0);  // d/ds_x = -d/dx
+  dsy = -(-fa * wx1 + fb * wx1 - fc * wx0 + fd * wx0);  // d/ds_y = -d/dy
+}
+
+// One wavefront's share of a 64 x 8 output tile: input-row blocks j = wu,
+// wu + 4, ... of the staged tile (row length LW), all tap columns, nk kernels
+// (nk <= NKmax).  Taps go through the scalar cache in bursts of 16.
+template <int NKmax>
+__device__ __forceinline__ void conv_accumulate(const float* smem, int LW, int lane, int wu, int nT,
+                                                int kw, int kts, int nk,
+                                                const float* __restrict__ kt,
+                                                float (&acc)[NKmax][kRYF]) {
+  for (int j = wu; j < nT; j += 4) {
+    const int t0 = 8 * j;
+    const float* trow = smem + t0 * LW + lane;
+    const float* ktj = kt + t0;
+    for (int b = 0; b < kw; ++b) {
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = trow[i * LW + b];
+#pragma unroll
+      for (int k = 0; k < NKmax; ++k) {
+        if (k < nk) {
+          const float4* kp = reinterpret_cast<const float4*>(ktj + ((int64_t)k * kw + b) * kts);
+          const float4 q0 = kp[0], q1 = kp[1], q2 = kp[2], q3 = kp[3];
+          const float kk[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int r = 0; r < kRYF; ++r) acc[k][r] = fmaf(kk[i - r + 7], v[i], acc[k][r]);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace apx

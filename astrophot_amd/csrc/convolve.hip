@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "capi_internal.h"
+#include "conv_common.h"
 
 namespace apx {
 
@@ -38,25 +39,6 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   return red[0] + red[1] + red[2] + red[3];
-}
-
-__device__ __forceinline__ void bilinear(const float* psf, int ph, int pw, int r, int c, double sx,
-                                         double sy, double& val, double& dsx, double& dsy) {
-  const int H2 = ph + 2, W2 = pw + 2;
-  const double x = (double)c - sx, y = (double)r - sy;
-  const int xf = (int)floor(x), yf = (int)floor(y);
-  const int x0 = min(max(xf, 0), W2 - 2), x1 = min(max(xf + 1, 1), W2 - 1);
-  const int y0 = min(max(yf, 0), H2 - 2), y1 = min(max(yf + 1, 1), H2 - 1);
-  auto at = [&](int yy, int xx) -> double {
-    return (yy >= 1 && yy <= ph && xx >= 1 && xx <= pw) ? (double)psf[(yy - 1) * pw + (xx - 1)]
-                                                        : 0.0;
-  };
-  const double fa = at(y0, x0), fb = at(y1, x0), fc = at(y0, x1), fd = at(y1, x1);
-  const double wx1 = (double)x1 - x, wx0 = x - (double)x0;
-  const double wy1 = (double)y1 - y, wy0 = y - (double)y0;
-  val = fa * wx1 * wy1 + fb * wx1 * wy0 + fc * wx0 * wy1 + fd * wx0 * wy0;
-  dsx = -(-fa * wy1 - fb * wy0 + fc * wy1 + fd * wy0);  // d/ds_x = -d/dx
-  dsy = -(-fa * wx1 + fb * wx1 - fc * wx0 + fd * wx0);  // d/ds_y = -d/dy
 }
 
 __global__ __launch_bounds__(256) void psf_shift_kernel(const float* psf, int ph, int pw, double sx,
@@ -181,11 +163,6 @@ __global__ __launch_bounds__(256) void convolve_kernel(apx_conv_job job, const f
 // the kernel) so every (input row, output row) pair finds its tap at
 // u = t - r + 7 with no bounds tests.  The four partial sums are combined in
 // LDS in a fixed order (deterministic).
-constexpr int kRYF = 8;      // output rows per workgroup (one per accumulator)
-constexpr int kFastMax = 89; // largest kernel side whose tile fits 64 KiB of LDS
-
-__host__ __device__ inline int fast_nt(int kh) { return (kh + kRYF - 1 + 7) / 8; }
-__host__ __device__ inline int fast_kts(int kh) { return 8 * fast_nt(kh) + 8; }
 
 __global__ __launch_bounds__(256) void conv_prep_kernel(const float* __restrict__ kernels, int nk,
                                                         int kh, int kw, int kts, float* kt) {
@@ -229,29 +206,7 @@ __global__ __launch_bounds__(256) void convolve_fast_kernel(apx_conv_job job,
       for (int r = 0; r < kRYF; ++r) acc[k][r] = 0.0f;
     // wave index made provably uniform so tap addresses go to the scalar unit
     const int wu = __builtin_amdgcn_readfirstlane(w);
-    for (int j = wu; j < nT; j += 4) {
-      const int t0 = 8 * j;
-      const float* trow = smem + t0 * LW + lane;
-      const float* ktj = kt + t0;
-      for (int b = 0; b < kw; ++b) {
-        float v[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = trow[i * LW + b];
-#pragma unroll
-        for (int k = 0; k < NKmax; ++k) {
-          if (k < nk) {
-            const float4* kp = reinterpret_cast<const float4*>(ktj + ((int64_t)k * kw + b) * kts);
-            const float4 q0 = kp[0], q1 = kp[1], q2 = kp[2], q3 = kp[3];
-            const float kk[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                                  q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
-#pragma unroll
-            for (int i = 0; i < 8; ++i)
-#pragma unroll
-              for (int r = 0; r < kRYF; ++r) acc[k][r] = fmaf(kk[i - r + 7], v[i], acc[k][r]);
-          }
-        }
-      }
-    }
+    conv_accumulate<NKmax>(smem, LW, lane, wu, nT, kw, kts, nk, kt, acc);
     // combine the four wavefronts' partial sums: red[w][k][r][lane]
     __syncthreads();
     float* red = smem;
@@ -314,9 +269,12 @@ int convolve(const apx_conv_job* jp, const float* planes, const float* kernels, 
   if (job.ow == 0 || job.oh == 0) return APX_OK;
   // every output reads inside the working grid
   const int ch = (job.kh - 1) / 2, cw = (job.kw - 1) / 2;
-  if (job.x0 - cw < 0 || job.y0 - ch < 0 || job.x0 + job.ow - 1 + cw >= job.nx ||
-      job.y0 + job.oh - 1 + ch >= job.ny)
+  if (!job.zero_outside &&
+      (job.x0 - cw < 0 || job.y0 - ch < 0 || job.x0 + job.ow - 1 + cw >= job.nx ||
+       job.y0 + job.oh - 1 + ch >= job.ny))
     return set_error(APX_E_INVALID, "convolve: output box reads outside the working grid");
+  if (job.x0 < 0 || job.y0 < 0 || job.x0 + job.ow > job.nx || job.y0 + job.oh > job.ny)
+    return set_error(APX_E_INVALID, "convolve: output box outside the working grid");
   if (job.img) {
     if (job.img_row < job.img_w || job.img_x0 < 0 || job.img_y0 < 0 ||
         job.img_x0 + job.ow > job.img_w || job.img_y0 + job.oh > job.img_h)

@@ -50,58 +50,66 @@ struct Prof {
 };
 
 template <typename T>
-__device__ inline Prof<T> make_prof(const apx_component& cp) {
+__device__ inline Prof<T> make_prof(int profile, const double* P, double fx, double fy,
+                                    double pixel_area, double softening, double q, double pa,
+                                    const double* prm) {
   Prof<T> p;
-  p.kind = cp.profile;
-  p.P00 = (T)cp.P[0];
-  p.P01 = (T)cp.P[1];
-  p.P10 = (T)cp.P[2];
-  p.P11 = (T)cp.P[3];
-  p.fx = (T)cp.frac[0];
-  p.fy = (T)cp.frac[1];
-  const double theta = -(cp.pa - kPi / 2.0);
+  p.kind = profile;
+  p.P00 = (T)P[0];
+  p.P01 = (T)P[1];
+  p.P10 = (T)P[2];
+  p.P11 = (T)P[3];
+  p.fx = (T)fx;
+  p.fy = (T)fy;
+  const double theta = -(pa - kPi / 2.0);
   p.c = (T)cos(theta);
   p.s = (T)sin(theta);
-  p.q = (T)cp.q;
-  p.invq = (T)(1.0 / cp.q);
-  p.qm = (T)(cp.q - 1.0 / cp.q);
-  p.soft2 = (T)(cp.softening * cp.softening);
+  p.q = (T)q;
+  p.invq = (T)(1.0 / q);
+  p.qm = (T)(q - 1.0 / q);
+  p.soft2 = (T)(softening * softening);
   p.a = p.inva = p.b = p.db = p.binva = p.inv_s = (T)0;
-  switch (cp.profile) {
+  switch (profile) {
     case APX_SERSIC: {  // prm: n, Re, Ie
-      const double n = cp.prm[0];
-      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[2]));
+      const double n = prm[0];
+      p.amp = (T)(pixel_area * pow(10.0, prm[2]));
       p.a = (T)n;
       p.inva = (T)(1.0 / n);
       p.b = (T)sersic_b(n);
       p.db = (T)sersic_db(n);
-      p.inv_s = (T)(1.0 / cp.prm[1]);  // 1/Re
+      p.inv_s = (T)(1.0 / prm[1]);  // 1/Re
       break;
     }
     case APX_EXPONENTIAL: {  // prm: Re, Ie ; b(1) from the Sersic polynomial
-      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[1]));
+      p.amp = (T)(pixel_area * pow(10.0, prm[1]));
       p.b = (T)sersic_b(1.0);
-      p.inv_s = (T)(1.0 / cp.prm[0]);
+      p.inv_s = (T)(1.0 / prm[0]);
       break;
     }
     case APX_GAUSSIAN: {  // prm: sigma, flux ; I0/sqrt(2 pi sigma^2)
-      const double sg = cp.prm[0];
-      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[1]) / sqrt(2.0 * kPi * sg * sg));
+      const double sg = prm[0];
+      p.amp = (T)(pixel_area * pow(10.0, prm[1]) / sqrt(2.0 * kPi * sg * sg));
       p.inv_s = (T)(1.0 / sg);
       break;
     }
     case APX_MOFFAT: {  // prm: n, Rd, I0
-      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[2]));
-      p.a = (T)cp.prm[0];
-      p.inv_s = (T)(1.0 / cp.prm[1]);
+      p.amp = (T)(pixel_area * pow(10.0, prm[2]));
+      p.a = (T)prm[0];
+      p.inv_s = (T)(1.0 / prm[1]);
       break;
     }
     default: {  // flat sky: prm F
-      p.amp = (T)(cp.pixel_area * pow(10.0, cp.prm[0]));
+      p.amp = (T)(pixel_area * pow(10.0, prm[0]));
       break;
     }
   }
   return p;
+}
+
+template <typename T>
+__device__ inline Prof<T> make_prof(const apx_component& cp) {
+  return make_prof<T>(cp.profile, cp.P, cp.frac[0], cp.frac[1], cp.pixel_area, cp.softening, cp.q,
+                      cp.pa, cp.prm);
 }
 
 // Value (times `scale`, the pixel-area ratio of the sampling level) and, when

@@ -21,6 +21,7 @@
 
 #include "capi_internal.h"
 #include "profile.h"
+#include "render_common.h"
 
 namespace apx {
 
@@ -36,6 +37,8 @@ struct RenderWs {
   double* partial;    // per-init-block sums
   int32_t* list;      // selected pixel indices
   float* centre;      // quad:N centre-node samples
+  uint8_t* flags;     // 1 = pixel selected for integration
+  double* vpart;      // per-integrate-block adjoint sums (8 per block)
 };
 
 __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
@@ -51,54 +54,15 @@ __host__ inline RenderWs carve(void* ws, int64_t npix) {
   w.reference = reinterpret_cast<double*>(b + 64);
   int64_t off = 256;
   w.partial = reinterpret_cast<double*>(b + off);
-  off += align256(8 * init_blocks(npix));
+  off += align256(8 * 8 * init_blocks(npix));
   w.list = reinterpret_cast<int32_t*>(b + off);
   off += align256(4 * npix);
   w.centre = reinterpret_cast<float*>(b + off);
+  off += align256(4 * npix);
+  w.flags = reinterpret_cast<uint8_t*>(b + off);
+  off += align256(npix);
+  w.vpart = reinterpret_cast<double*>(b + off);
   return w;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// offsets of a working pixel from the centre
-template <typename T>
-__device__ __forceinline__ void pixel_xy(const Prof<T>& p, const apx_component& cp, int i, int j,
-                                         T& X, T& Y) {
-  const T di = (T)(i - cp.ci);
-  const T dj = (T)(j - cp.cj);
-  X = p.P00 * di + p.P01 * dj + p.fx;
-  Y = p.P10 * di + p.P11 * dj + p.fy;
-}
-
-// quad:N sample of one pixel (operations.py:123-147): returns the weighted
-// sum, `centre` the middle node; accumulates weighted slot gradients in g.
-template <typename T, bool GRAD>
-__device__ T quad_sample(const Prof<T>& p, const apx_component& cp, T X, T Y, int n,
-                         const double* node, const double* weight, T scale, T hP00, T hP01,
-                         T hP10, T hP11, T* g, T& centre) {
-  T res = (T)0;
-  T gl[APX_NSLOT];
-  if (GRAD) {
-#pragma unroll
-    for (int k = 0; k < APX_NSLOT; ++k) g[k] = (T)0;
-  }
-  for (int k = 0; k < n * n; ++k) {
-    const T u = (T)node[k % n], v = (T)node[k / n];
-    const T w = (T)(weight[k % n] * weight[k / n] * 0.25);
-    const T val = eval_profile<T, GRAD>(p, X + hP00 * u + hP01 * v, Y + hP10 * u + hP11 * v,
-                                        scale, gl);
-    res += w * val;
-    if (k == (n * n) / 2) centre = val;
-    if (GRAD) {
-#pragma unroll
-      for (int s = 0; s < APX_NSLOT; ++s) g[s] += w * gl[s];
-    }
-  }
-  return res;
 }
 
 template <bool GRAD>
@@ -212,6 +176,7 @@ __global__ __launch_bounds__(256) void render_select_kernel(apx_component cp, co
       sel = select_f64(cp, p, i, j, thr);
     }
   }
+  if (pix < npix) ws.flags[pix] = sel ? 1 : 0;
   const uint64_t ballot = __ballot(sel);
   if (ballot) {
     const int lane = threadIdx.x & 63;
@@ -236,99 +201,10 @@ __global__ __launch_bounds__(256) void render_integrate_kernel(apx_component cp,
   const int64_t npix = (int64_t)cp.nx * cp.ny;
   const Prof<double> p = make_prof<double>(cp);
   const double thr1 = cp.tolerance * (cp.mean_reference ? ws.reference[0] : cp.reference);
-  const int n = cp.quad_level, nn = n * n, g = cp.gridding, gg = g * g;
-  const double ig = 1.0 / (double)g;
-  const double sub0 = -(double)(g - 1) / (2.0 * g), subd = 1.0 / (double)g;  // linspace step
-  // node offsets scale with the level's pixelscale: P / g^(level-1) / 2
   for (int k = wave; k < count; k += nwaves) {
     const int32_t pix = ws.list[k];
-    const int i = pix % cp.nx, j = pix / cp.nx;
-    double X0, Y0;
-    pixel_xy(p, cp, i, j, X0, Y0);
-    double acc[1 + APX_NSLOT];
-#pragma unroll
-    for (int s = 0; s <= APX_NSLOT; ++s) acc[s] = 0.0;
-    double gl[APX_NSLOT];
-    // ---- level 1 (area 1): lane < nn is a node
-    double v1 = 0.0;
-    if (lane < nn) {
-      const double u = cp.gl_node[lane % n], v = cp.gl_node[lane / n];
-      const double w = cp.gl_weight[lane % n] * cp.gl_weight[lane / n] * 0.25;
-      const double x = X0 + 0.5 * (p.P00 * u + p.P01 * v), y = Y0 + 0.5 * (p.P10 * u + p.P11 * v);
-      v1 = eval_profile<double, GRAD>(p, x, y, 1.0, gl);
-      acc[0] = w * v1;
-      if (GRAD) {
-#pragma unroll
-        for (int s = 0; s < APX_NSLOT; ++s) acc[1 + s] = w * gl[s];
-      }
-    }
-    const double res1 = wave_sum(acc[0]);
-    const double ref1 = __shfl(v1, nn / 2, 64);
     double out[1 + APX_NSLOT];
-    if (cp.max_depth <= 1 || !(fabs(res1 - ref1) > thr1)) {
-      out[0] = res1;
-      if (GRAD) {
-#pragma unroll
-        for (int s = 0; s < APX_NSLOT; ++s) out[1 + s] = wave_sum(acc[1 + s]);
-      }
-    } else {
-      // ---- level 2 (pixelscale P/g, area 1/g^2): lane < gg is a sub-pixel
-      const double sc2 = ig * ig, thr2 = thr1 * (double)gg;
-      double c2[1 + APX_NSLOT];
-#pragma unroll
-      for (int s = 0; s <= APX_NSLOT; ++s) c2[s] = 0.0;
-      bool deeper = false;
-      double x2 = 0.0, y2 = 0.0;
-      if (lane < gg) {
-        const double a = sub0 + subd * (lane % g), b = sub0 + subd * (lane / g);
-        x2 = X0 + p.P00 * a + p.P01 * b;
-        y2 = Y0 + p.P10 * a + p.P11 * b;
-        double gs[APX_NSLOT], centre = 0.0;
-        const double r2 = quad_sample<double, GRAD>(p, cp, x2, y2, n, cp.gl_node, cp.gl_weight,
-                                                    sc2, 0.5 * ig * p.P00, 0.5 * ig * p.P01,
-                                                    0.5 * ig * p.P10, 0.5 * ig * p.P11, gs, centre);
-        deeper = cp.max_depth > 2 && fabs(r2 - centre) > thr2;
-        if (!deeper) {
-          c2[0] = r2;
-          if (GRAD) {
-#pragma unroll
-            for (int s = 0; s < APX_NSLOT; ++s) c2[1 + s] = gs[s];
-          }
-        }
-      }
-#pragma unroll
-      for (int s = 0; s <= APX_NSLOT; ++s)
-        if (GRAD || s == 0) out[s] = wave_sum(c2[s]);
-      // ---- level 3 (pixelscale P/g^2, area 1/g^4): whole wave per flagged sub-pixel
-      uint64_t todo = __ballot(deeper);
-      const double sc3 = sc2 * ig * ig;
-      while (todo) {
-        const int sp = __ffsll((unsigned long long)todo) - 1;
-        todo &= todo - 1;
-        const double xs = __shfl(x2, sp, 64), ys = __shfl(y2, sp, 64);
-        double c3[1 + APX_NSLOT];
-#pragma unroll
-        for (int s = 0; s <= APX_NSLOT; ++s) c3[s] = 0.0;
-        for (int e = lane; e < gg * nn; e += 64) {
-          const int t = e / nn, kk = e % nn;
-          const double a = (sub0 + subd * (t % g)) * ig, b = (sub0 + subd * (t / g)) * ig;
-          const double u = cp.gl_node[kk % n], v = cp.gl_node[kk / n];
-          const double w = cp.gl_weight[kk % n] * cp.gl_weight[kk / n] * 0.25;
-          const double hu = 0.5 * ig * ig * u, hv = 0.5 * ig * ig * v;
-          const double x = xs + p.P00 * (a + hu) + p.P01 * (b + hv);
-          const double y = ys + p.P10 * (a + hu) + p.P11 * (b + hv);
-          const double val = eval_profile<double, GRAD>(p, x, y, sc3, gl);
-          c3[0] += w * val;
-          if (GRAD) {
-#pragma unroll
-            for (int s = 0; s < APX_NSLOT; ++s) c3[1 + s] += w * gl[s];
-          }
-        }
-#pragma unroll
-        for (int s = 0; s <= APX_NSLOT; ++s)
-          if (GRAD || s == 0) out[s] += wave_sum(c3[s]);
-      }
-    }
+    integrate_pixel<GRAD>(cp, p, cp.nx, cp.ci, cp.cj, pix, thr1, lane, out);
     if (lane == 0) {
       planes[pix] = (float)out[0];
       if (GRAD) {
@@ -341,12 +217,91 @@ __global__ __launch_bounds__(256) void render_integrate_kernel(apx_component cp,
   }
 }
 
+// Adjoint of the render: sum over working pixels of B[q] * d deep[q] / d slot
+// for the slots chan_slot[1 .. nchan-1].  Unselected pixels use the float32
+// analytic partials (what the Jacobian channels hold); selected pixels
+// recompute the float64 quadrature (render_vjp_integrate_kernel).
+__global__ __launch_bounds__(256) void render_vjp_kernel(apx_component cp,
+                                                         const float* __restrict__ B, RenderWs ws,
+                                                         int use_flags) {
+  __shared__ double red[4][8];
+  const int64_t npix = (int64_t)cp.nx * cp.ny;
+  const int64_t pix = (int64_t)blockIdx.x * kInitBlock + threadIdx.x;
+  double acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+  if (pix < npix && !(use_flags && ws.flags[pix])) {
+    const Prof<float> p = make_prof<float>(cp);
+    const int i = (int)(pix % cp.nx), j = (int)(pix / cp.nx);
+    float X, Y, g[APX_NSLOT];
+    pixel_xy(p, cp, i, j, X, Y);
+    if (cp.sampling == APX_SAMPLE_QUAD) {
+      float centre;
+      quad_sample<float, true>(p, cp, X, Y, cp.sample_quad, cp.sq_node, cp.sq_weight, 1.0f,
+                               0.5f * p.P00, 0.5f * p.P01, 0.5f * p.P10, 0.5f * p.P11, g, centre);
+    } else {
+      eval_profile<float, true>(p, X, Y, 1.0f, g);
+    }
+    const double w = (double)B[pix];
+    for (int c = 1; c < cp.nchan; ++c) acc[c] = w * (double)g[cp.chan_slot[c]];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 1; c < 8; ++c) {
+    const double t = wave_sum(acc[c]);
+    if (lane == 0) red[wv][c] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x >= 1 && threadIdx.x < 8)
+    ws.partial[blockIdx.x * 8 + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void render_vjp_integrate_kernel(apx_component cp,
+                                                                   const float* __restrict__ B,
+                                                                   RenderWs ws) {
+  __shared__ double red[4][8];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wave = (int)(blockIdx.x * 4 + wv);
+  const int nwaves = (int)(gridDim.x * 4);
+  const int count = ws.count[0];
+  const Prof<double> p = make_prof<double>(cp);
+  const double thr1 = cp.tolerance * (cp.mean_reference ? ws.reference[0] : cp.reference);
+  double acc[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+  for (int k = wave; k < count; k += nwaves) {
+    const int32_t pix = ws.list[k];
+    double out[1 + APX_NSLOT];
+    integrate_pixel<true>(cp, p, cp.nx, cp.ci, cp.cj, pix, thr1, lane, out);
+    const double w = (double)B[pix];
+    for (int c = 1; c < cp.nchan; ++c) acc[c] += w * out[1 + cp.chan_slot[c]];
+  }
+  if (lane == 0)
+    for (int c = 0; c < 8; ++c) red[wv][c] = acc[c];
+  __syncthreads();
+  if (threadIdx.x >= 1 && threadIdx.x < 8)
+    ws.vpart[blockIdx.x * 8 + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(64) void render_vjp_final_kernel(apx_component cp, RenderWs ws,
+                                                              int nb1, int nb2, double* grad) {
+  const int c = threadIdx.x;
+  if (c < 1 || c >= cp.nchan) return;
+  double t = 0.0;
+  for (int b = 0; b < nb1; ++b) t += ws.partial[b * 8 + c];
+  for (int b = 0; b < nb2; ++b) t += ws.vpart[b * 8 + c];
+  grad[c - 1] = t * cp.dvdr[cp.chan_slot[c]];
+}
+
 __global__ void zero_count_kernel(RenderWs ws) { ws.count[0] = 0; }
 
 }  // namespace
 
 int64_t render_workspace_bytes(int64_t npix) {
-  return 256 + align256(8 * init_blocks(npix)) + align256(4 * npix) + align256(4 * npix);
+  return 256 + align256(8 * 8 * init_blocks(npix)) + align256(4 * npix) + align256(4 * npix) +
+         align256(npix) + align256(8 * 8 * (kIntegrateWaves / 4));
 }
 
 int render_component(const apx_component* cp_in, float* planes, void* workspace, int64_t ws_bytes,
@@ -401,6 +356,32 @@ int render_component(const apx_component* cp_in, float* planes, void* workspace,
                          planes, ws);
   }
   return check_launch("render");
+}
+
+int render_vjp(const apx_component* cp_in, const float* weights, float* value_plane, double* grad,
+               void* workspace, int64_t ws_bytes, hipStream_t stream) {
+  if (!cp_in || !weights || !value_plane || !grad)
+    return set_error(APX_E_INVALID, "render_vjp: null pointer");
+  apx_component cp = *cp_in;
+  if (cp.nchan < 2) return set_error(APX_E_INVALID, "render_vjp: no slots requested (nchan < 2)");
+  // the value render (supersamples recomputed) establishes the selection
+  apx_component cv = cp;
+  cv.nchan = 1;
+  int rc = render_component(&cv, value_plane, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  const int64_t npix = (int64_t)cp.nx * cp.ny;
+  const RenderWs ws = carve(workspace, npix);
+  const int64_t nb = init_blocks(npix);
+  hipLaunchKernelGGL(render_vjp_kernel, dim3((unsigned)nb), dim3(kInitBlock), 0, stream, cp, weights,
+                     ws, cp.integrate);
+  int nb2 = 0;
+  if (cp.integrate) {
+    nb2 = (int)std::min<int64_t>((npix + 3) / 4, kIntegrateWaves / 4);
+    hipLaunchKernelGGL(render_vjp_integrate_kernel, dim3(nb2), dim3(256), 0, stream, cp, weights, ws);
+  }
+  hipLaunchKernelGGL(render_vjp_final_kernel, dim3(1), dim3(64), 0, stream, cp, ws, (int)nb, nb2,
+                     grad);
+  return check_launch("render_vjp");
 }
 
 int render_selected_count(const void* workspace, hipStream_t stream, int32_t* count) {

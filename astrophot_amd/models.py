@@ -26,7 +26,7 @@ from .image import Jacobian_Image, Model_Image, PSF_Image, Target_Image, Window
 from .param import Param_Mask, Parameter_Node
 
 __all__ = ["AstroPhot_Model", "Component_Model", "Galaxy_Model", "Sersic_Galaxy", "Exponential_Galaxy",
-           "Gaussian_Galaxy", "Moffat_Galaxy", "Sky_Model", "Flat_Sky", "Group_Model"]
+           "Gaussian_Galaxy", "Moffat_Galaxy", "Sky_Model", "Flat_Sky", "Group_Model", "differentiable_sample"]
 
 
 def _all_subclasses(cls):
@@ -272,6 +272,30 @@ class Component_Model(AstroPhot_Model):
         _engine.render(self, self._values(),
                        _engine.RenderRequest(window=window, jac=jac, slot_cols=slot_cols, dvdr=dvdr))
 
+    def vjp(self, G, parameters: Optional[torch.Tensor] = None, as_representation: bool = True,
+            window: Optional[Window] = None):
+        """Vector-Jacobian product: sum over the model image of G * d image /
+        d parameter, for this model's flat parameter vector (float64, device).
+        ``G`` is a float32 image over ``window`` (default: this model's window).
+        Runs the hand-written adjoint (``_engine.vjp``); nothing is stored from
+        the forward pass."""
+        if parameters is not None:
+            if as_representation:
+                self.parameters.vector_set_representation(parameters)
+            else:
+                self.parameters.vector_set_values(parameters)
+        G_window = self.window if window is None else window
+        G = torch.as_tensor(G, device=AP_config.ap_device).to(torch.float32).contiguous()
+        n = len(self.parameters.vector_identities())
+        if n == 0:
+            return torch.zeros(0, dtype=torch.float64, device=AP_config.ap_device)
+        slots = self._free_slots()
+        slot_cols = {s: k for k, (s, _) in enumerate(slots)}
+        dvdr = {s: (d if as_representation else 1.0) for s, d in slots}
+        sample_window = self.window & G_window
+        out, col_index = _engine.vjp(self, self._values(), sample_window, G, G_window, slot_cols, dvdr)
+        return out[[col_index[k] for k in range(n)]]
+
     def get_state(self):
         state = {"name": self.name, "model_type": self.model_type, "window": self.window.get_state(),
                  "parameters": self.parameters.get_state()}
@@ -471,8 +495,56 @@ class Group_Model(AstroPhot_Model):
             model.jacobian(as_representation=as_representation, pass_jacobian=jac, window=window)
         return jac
 
+    def vjp(self, G, parameters: Optional[torch.Tensor] = None, as_representation: bool = True,
+            window: Optional[Window] = None):
+        """Vector-Jacobian product of the group image (see Component_Model.vjp);
+        components are sampled on the group window as in ``sample``."""
+        if parameters is not None:
+            if as_representation:
+                self.parameters.vector_set_representation(parameters)
+            else:
+                self.parameters.vector_set_values(parameters)
+        G_window = self.window if window is None else window
+        G = torch.as_tensor(G, device=AP_config.ap_device).to(torch.float32).contiguous()
+        parts = []
+        for model in self.models.values():
+            n = len(model.parameters.vector_identities())
+            if n == 0:
+                continue
+            slots = model._free_slots()
+            slot_cols = {sl: k for k, (sl, _) in enumerate(slots)}
+            dvdr = {sl: (d if as_representation else 1.0) for sl, d in slots}
+            out, col_index = _engine.vjp(model, model._values(), G_window, G, G_window, slot_cols, dvdr,
+                                         region=model.window & G_window)
+            parts.append(out[[col_index[k] for k in range(n)]])
+        return torch.cat(parts) if parts else torch.zeros(0, dtype=torch.float64, device=AP_config.ap_device)
+
     def fit_mask(self):
         return torch.zeros_like(self.target[self.window].mask)
 
     def __iter__(self):
         return iter(self.models.values())
+
+
+class _RenderFunction(torch.autograd.Function):
+    """model image as a differentiable function of the representation vector;
+    backward is the hand-written adjoint (``model.vjp``)."""
+
+    @staticmethod
+    def forward(ctx, rep, model):
+        ctx.model = model
+        ctx.save_for_backward(rep)
+        img = model(parameters=rep.detach().to("cpu", torch.float64), as_representation=True)
+        return img.data.clone()
+
+    @staticmethod
+    def backward(ctx, G):
+        (rep,) = ctx.saved_tensors
+        g = ctx.model.vjp(G, parameters=rep.detach().to("cpu", torch.float64), as_representation=True)
+        return g.to(device=rep.device, dtype=rep.dtype), None
+
+
+def differentiable_sample(model, rep: torch.Tensor) -> torch.Tensor:
+    """``model(parameters=rep, as_representation=True).data`` with autograd
+    support through the gfx950 adjoint kernels."""
+    return _RenderFunction.apply(rep, model)

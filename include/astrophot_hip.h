@@ -107,6 +107,16 @@ int64_t apx_render_workspace_bytes(int64_t npix);
 int apx_render_component(const apx_component* comp, float* planes, void* workspace,
                          int64_t workspace_bytes, void* stream);
 
+/* Adjoint of apx_render_component: grad[c-1] = sum over working pixels q of
+ * weights[q] * d value[q] / d rep(chan_slot[c]) for c = 1 .. nchan-1, float64,
+ * device.  `weights` (ny x nx) is the upstream gradient already carried back
+ * through the convolution.  The value render is recomputed into
+ * `value_plane` (ny x nx); supersampled pixels are re-integrated in float64,
+ * nothing of the forward pass is stored.
+ * Replaces the reverse-mode path of models/_model_methods.py:82-184. */
+int apx_render_vjp(const apx_component* comp, const float* weights, float* value_plane,
+                   double* grad, void* workspace, int64_t workspace_bytes, void* stream);
+
 /* Number of pixels the last apx_render_component on `workspace` integrated
  * (copies one int to the host; synchronises the stream). */
 int apx_render_selected_count(const void* workspace, void* stream, int32_t* count);
@@ -138,6 +148,8 @@ typedef struct apx_conv_job {
   int32_t jac_x0, jac_y0;  /* jac pixel of output (0, 0) */
   int32_t chan_col[APX_MAX_CHAN]; /* jac column of render plane c (c >= 1) */
   int32_t dk_col[2];       /* jac columns of conv(plane 0, dK/dc), -1 = none */
+  int32_t zero_outside;    /* 1: taps reaching outside the planes read zero
+                              (adjoint pass); 0: the box must stay inside */
 } apx_conv_job;
 
 /* Workspace (bytes) apx_convolve needs for a kh x kw kernel. */
@@ -176,6 +188,79 @@ int apx_rpp(const float* J, const float* m0, const float* m1, const float* weigh
  * row k is not positive (h is then NaN). */
 int apx_lm_solve(const double* hess, const double* grad, int32_t np, double L, double* h,
                  int32_t* status, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Batched independent fits (north-star configuration C2): every cutout holds
+ * one Sersic galaxy convolved with its own PSF (psf_mode "full", bilinear
+ * sub-pixel shift, midpoint sampling, threshold integration) and is fitted by
+ * the reference's Levenberg-Marquardt (fit/lm.py:248-493, defaults L0=1,
+ * Lup=11, Ldn=9, 10 sub-steps, curvature limit 1, acceleration 0).  All
+ * fits advance together on the device: one "tick" is one LM sub-step of every
+ * fit that is inside a step, plus the Jacobian pass of every fit that starts
+ * one.  A tick is a fixed sequence of launches over device-built work lists,
+ * so it can be captured in a HIP graph and replayed.
+ * Parameter vector per fit (representation space, float64):
+ *   [center_x, center_y, q, PA, n, Re, Ie] with the maps of
+ *   galaxy_model_object.py:44-53 and sersic_model.py:61-66.
+ * ------------------------------------------------------------------------- */
+#define APX_BATCH_NP 7
+
+typedef struct apx_batch_galaxy {
+  int32_t nx, ny;     /* cutout size */
+  int32_t ph, pw;     /* PSF size (odd) */
+  int64_t pix_off;    /* first cutout pixel in data / weight / model / jac */
+  int64_t psf_off;    /* first PSF value in psf */
+  int64_t work_off;   /* first working pixel (cutout + PSF border) in planes */
+  int64_t tap_off;    /* first float of this fit's re-arranged taps */
+  int64_t chunk_off;  /* first 256-pixel working chunk (global index) */
+  int64_t tile_off;   /* first 64 x 8 output tile (global index) */
+  double origin[2];   /* plane coordinates of the cutout corner (pixel -0.5,-0.5) */
+} apx_batch_galaxy;
+
+typedef struct apx_batch {
+  int32_t ngal;
+  int32_t max_kernel;          /* largest PSF side + 2 in the batch */
+  int64_t npix, nwork, nchunk, ntile, ntap;  /* totals (set by apx_batch_plan) */
+  double P[4];                 /* pixelscale shared by the batch (row-major) */
+  double softening, tolerance; /* softening, sampling_tolerance */
+  int32_t quad_level, gridding, max_depth;   /* integrate_* */
+  int32_t max_iter;            /* LM max_iter */
+  double relative_tolerance;   /* LM relative_tolerance */
+  double L0;                   /* initial damping */
+  /* device buffers, allocated by the caller with the sizes apx_batch_plan sets */
+  const apx_batch_galaxy* gal; /* ngal */
+  const float* data;           /* npix */
+  const float* weight;         /* npix, 1/variance */
+  const float* psf;            /* sum of ph * pw */
+  float* model0;               /* npix: model at the current state */
+  float* model1;               /* npix: probe / trial model */
+  float* jac;                  /* npix x APX_BATCH_NP, pixel-major */
+  float* planes;               /* 6 x nwork */
+  float* taps;                 /* ntap */
+  void* state;                 /* ngal x apx_batch_state_bytes() */
+  void* scratch;               /* apx_batch_scratch_bytes(batch) */
+} apx_batch;
+
+/* Fill the offsets of gal[] (host array) and the totals of *batch. */
+int apx_batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* batch);
+int64_t apx_batch_state_bytes(void);
+int64_t apx_batch_scratch_bytes(const apx_batch* batch);
+
+/* Render every cutout at rep (device, ngal x 7 float64) into out (npix). */
+int apx_batch_render(const apx_batch* batch, const double* rep, float* out, void* stream);
+
+/* Start the fits from rep0 (device, ngal x 7). */
+int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream);
+
+/* One LM tick of every unfinished fit. */
+int apx_batch_tick(const apx_batch* batch, void* stream);
+
+/* Results (device arrays): best state (res(), fit/base.py:104-117), its chi2
+ * (reduced), status (0 running, 1 success, 2 success by immobility, 3 fail:
+ * no improving step, 4 fail: max iterations), iterations, and the number of
+ * finished fits in done[0]. */
+int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
+                      int32_t* iterations, int32_t* done, void* stream);
 
 /* Last error message of the calling thread. */
 const char* apx_last_error(void);

@@ -1,0 +1,76 @@
+"""End-to-end fit.LM parity: the GPU path against the oracle's float64 LM on
+the same seeded data (the C0 setup: 100x100 Sersic + flat sky, 25x25 Gaussian
+PSF sigma 1.5, variance = model/5 + 0.01, truth rendered with quad:5, start
++-10 %).
+
+Bars: fitted parameters within 1e-4 relative (absolute 1e-4 for values near
+zero) and reduced chi^2 within 1e-6 relative; both fits must report success.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import astrophot_amd as ap
+from oracle import hotpath as H
+
+pytestmark = pytest.mark.gpu
+
+PARAM_RTOL = 1e-4
+CHI2_RTOL = 1e-6
+
+
+def c0_problem(seed=0, n=100, Re=12.0):
+    truth = {"center": [n / 2 + 0.3, n / 2 - 0.3], "q": 0.7, "PA": 0.6, "n": 2.0, "Re": Re, "Ie": 1.0}
+    F = math.log10(0.5)
+    psf = H.gaussian_psf(1.5, 25, 1.0)
+    g = H.Grid(n, n)
+    ot = H.Model([H.Component("sersic", g.copy(), truth, psf=torch.tensor(psf), psf_mode="full",
+                              sampling_mode="quad:5"),
+                  H.Component("flat_sky", g.copy(), {"F": F})], window=g.copy())
+    img = ot.forward(ot.vector_representation()).numpy()
+    rng = np.random.default_rng(seed)
+    var = img / 5.0 + 0.01
+    data = img + rng.normal(size=img.shape) * np.sqrt(var)
+    return truth, F, psf, data, var, rng
+
+
+def test_lm_c0_matches_oracle():
+    truth, F, psf, data, var, rng = c0_problem()
+    n = data.shape[0]
+    g = H.Grid(n, n)
+    om = H.Model([H.Component("sersic", g.copy(), truth, psf=torch.tensor(psf), psf_mode="full"),
+                  H.Component("flat_sky", g.copy(), {"F": F})], window=g.copy())
+    vals = om.vector_values().numpy()
+    start_vals = vals * (1 + rng.uniform(-0.1, 0.1, len(vals)))
+    # the start point in representation space, built by each side from the same values
+    oc = om.components
+    o_start = torch.cat([oc[0].val_to_rep(torch.tensor(start_vals[:7])), oc[1].val_to_rep(torch.tensor(start_vals[7:]))])
+    lm_o = H.LM(om, torch.tensor(data), weight=torch.tensor(1 / var), initial_state=o_start).fit()
+
+    tgt = ap.image.Target_Image(data=data, pixelscale=1.0, variance=var,
+                                psf=ap.image.PSF_Image(data=psf, pixelscale=1.0))
+    s = ap.models.AstroPhot_Model(name="c0 sersic", model_type="sersic galaxy model", target=tgt,
+                                  parameters=truth, psf_mode="full")
+    k = ap.models.AstroPhot_Model(name="c0 sky", model_type="flat sky model", target=tgt, parameters={"F": F})
+    grp = ap.models.AstroPhot_Model(name="c0", model_type="group model", models=[s, k], target=tgt)
+    grp.parameters.vector_set_values(torch.tensor(start_vals))
+    start = grp.parameters.vector_representation()
+    assert torch.allclose(start, o_start, rtol=1e-12, atol=1e-12)
+    lm_g = ap.fit.LM(grp, initial_state=start).fit()
+
+    assert lm_o.message.startswith("success"), lm_o.message
+    assert lm_g.message.startswith("success"), lm_g.message
+    v_o = torch.cat([oc[0].rep_to_values(torch.as_tensor(lm_o.res()[:7]))[k].reshape(-1) for k in
+                     ("center", "q", "PA", "n", "Re", "Ie")] + [torch.as_tensor(lm_o.res()[7:])])
+    v_g = grp.parameters.vector_values()
+    err = ((v_g - v_o).abs() / v_o.abs().clamp_min(1.0)).max().item()
+    assert err < PARAM_RTOL, (err, v_g, v_o)
+    chi_o = min(lm_o.loss_history)
+    chi_g = min(lm_g.loss_history)
+    # the GPU chi^2 re-evaluated by the oracle at the GPU's parameters, and the GPU's own value
+    chi_og = lm_o._chi2(om.forward(torch.as_tensor(lm_g.res())).reshape(-1)).item()
+    assert abs(chi_g - chi_o) / chi_o < CHI2_RTOL, (chi_g, chi_o)
+    assert abs(chi_og - chi_o) / chi_o < CHI2_RTOL, (chi_og, chi_o)

@@ -37,13 +37,16 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     """ctypes mirrors of apx_component / apx_conv_job list every header field in order."""
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
-    for cname, cls in (("apx_component", L.Component), ("apx_conv_job", L.ConvJob)):
+    from astrophot_amd import batch as B
+    for cname, cls in (("apx_component", L.Component), ("apx_conv_job", L.ConvJob),
+                       ("apx_batch_galaxy", B.Galaxy), ("apx_batch", B.Batch)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), text, re.S).group(1)
         fields = []
         for decl in body.split(";"):
             decl = decl.strip()
             if not decl:
                 continue
+            decl = re.sub(r"^const\s+", "", decl)
             names = decl.split(None, 1)[1].replace("*", " ")
             for n in names.split(","):
                 fields.append(re.sub(r"\[.*\]", "", n).strip())

@@ -60,7 +60,8 @@ SIGNATURES = {
     "apx_batch_plan": (ctypes.c_int, [ctypes.POINTER(Galaxy), ctypes.c_int32, ctypes.POINTER(Batch)]),
     "apx_batch_state_bytes": (ctypes.c_int64, []),
     "apx_batch_scratch_bytes": (ctypes.c_int64, [ctypes.POINTER(Batch)]),
-    "apx_batch_render": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "apx_batch_render": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.c_void_p]),
     "apx_batch_start": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
     "apx_batch_tick": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p]),
     "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 6),
@@ -107,7 +108,7 @@ class BatchFit:
 
     def __init__(self, cutouts: List[Cutout], pixelscale=1.0, max_iter=100, relative_tolerance=1e-5, L0=1.0,
                  sampling_tolerance=1e-2, softening=1e-3, integrate_quad_level=3, integrate_gridding=5,
-                 integrate_max_depth=3):
+                 integrate_max_depth=3, data=None, variance=None):
         self.lib = L.load()
         dev = AP_config.ap_device
         ng = len(cutouts)
@@ -116,8 +117,8 @@ class BatchFit:
         self.cutouts = cutouts
         gal = (Galaxy * ng)()
         for k, c in enumerate(cutouts):
-            ny, nx = c.data.shape
-            ph, pw = c.psf.shape
+            ny, nx = np.shape(c.data)
+            ph, pw = np.shape(c.psf)
             gal[k].nx, gal[k].ny, gal[k].ph, gal[k].pw = nx, ny, ph, pw
             gal[k].origin[0], gal[k].origin[1] = float(c.origin[0]), float(c.origin[1])
         b = Batch()
@@ -132,9 +133,16 @@ class BatchFit:
         self.gal_host = gal
         f32 = dict(dtype=torch.float32, device=dev)
         self.gal_dev = torch.frombuffer(bytearray(gal), dtype=torch.uint8).to(dev)
-        self.data = torch.cat([torch.as_tensor(c.data, dtype=torch.float32).reshape(-1) for c in cutouts]).to(dev)
-        self.weight = torch.cat([(1.0 / torch.as_tensor(c.variance, dtype=torch.float64)).to(torch.float32).reshape(-1)
-                                 for c in cutouts]).to(dev)
+        if data is not None:  # flat device arrays in cutout order
+            self.data = torch.as_tensor(data, dtype=torch.float32, device=dev).reshape(-1).contiguous()
+            self.weight = (1.0 / torch.as_tensor(variance, device=dev).double()).to(torch.float32).reshape(-1)
+            if self.data.numel() != b.npix or self.weight.numel() != b.npix:
+                raise SpecificationConflict("flat data / variance do not match the cutout sizes")
+        else:
+            self.data = torch.cat([torch.as_tensor(c.data, dtype=torch.float32).reshape(-1)
+                                   for c in cutouts]).to(dev)
+            self.weight = torch.cat([(1.0 / torch.as_tensor(c.variance, dtype=torch.float64)).to(torch.float32)
+                                     .reshape(-1) for c in cutouts]).to(dev)
         self.psf = torch.cat([torch.as_tensor(c.psf, dtype=torch.float32).reshape(-1) for c in cutouts]).to(dev)
         self.model0 = torch.zeros(b.npix, **f32)
         self.model1 = torch.zeros(b.npix, **f32)
@@ -159,13 +167,23 @@ class BatchFit:
         self.ticks = 0
 
     # -- rendering --------------------------------------------------------
-    def render(self, rep) -> torch.Tensor:
+    def render(self, rep, out=None, parts=3) -> torch.Tensor:
         """All cutouts rendered at ``rep`` (ngal x 7, representation) into one
-        flat float32 tensor (cutouts concatenated row-major)."""
+        flat float32 tensor (cutouts concatenated row-major).  ``parts``: 1 =
+        working planes only, 2 = convolution of the planes of a previous
+        parts=1 call, 3 = both."""
         rep = torch.as_tensor(rep, dtype=torch.float64, device=self.data.device).contiguous()
-        out = torch.zeros(self.npix, dtype=torch.float32, device=self.data.device)
-        L.check(self.lib.apx_batch_render(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), L.stream()))
+        self._render_rep = rep
+        if out is None:
+            out = torch.zeros(self.npix, dtype=torch.float32, device=self.data.device)
+        L.check(self.lib.apx_batch_render(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), parts,
+                                          L.stream()))
         return out
+
+    def conv_flops(self) -> float:
+        """Algorithmic flops of one value convolution of every cutout:
+        sum of 2 (ph+2)(pw+2) nx ny."""
+        return float(sum(2.0 * (g.ph + 2) * (g.pw + 2) * g.nx * g.ny for g in self.gal_host))
 
     def split(self, flat: torch.Tensor) -> List[torch.Tensor]:
         out = []

@@ -771,9 +771,33 @@ __global__ void results_kernel(apx_batch b, double* rep, double* chi2, int32_t* 
 
 __global__ void zero_int_kernel(int32_t* p) { p[0] = 0; }
 
-// one render stage over the fits active in `stage`
-int run_stage(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
-              float* out, hipStream_t stream) {
+// Gauss-Legendre table on [-1, 1] by Newton iteration on P_n (the table
+// scipy.special.roots_legendre returns, operations.py:110)
+void gauss_legendre(int n, double* node, double* weight) {
+  for (int i = 0; i < n; ++i) {
+    double x = cos(kPi * (i + 0.75) / (n + 0.5));
+    double dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      const double pn = n == 1 ? x : p1, pm = n == 1 ? 1.0 : p0;
+      dp = n * (x * pn - pm) / (x * x - 1.0);
+      const double dx = pn / dp;
+      x -= dx;
+      if (fabs(dx) < 1e-16) break;
+    }
+    node[n - 1 - i] = x;
+    weight[n - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+  }
+}
+
+// work lists + render planes of the fits active in `stage`
+void render_part(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
+                 hipStream_t stream) {
   const bool jac = stage == ST_JAC;
   const int gb = (b.ngal + 255) / 256;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, s.cnt);
@@ -790,34 +814,16 @@ int run_stage(const apx_batch& b, const Scratch& s, int stage, const double* rep
   cfg.quad_level = b.quad_level;
   cfg.gridding = b.gridding;
   cfg.max_depth = b.max_depth;
-  {
-    // Gauss-Legendre table on [-1, 1] by Newton iteration on P_n (the table
-    // scipy.special.roots_legendre returns, operations.py:110)
-    const int n = b.quad_level;
-    for (int i = 0; i < n; ++i) {
-      double x = cos(kPi * (i + 0.75) / (n + 0.5));
-      double dp = 1.0;
-      for (int it = 0; it < 100; ++it) {
-        double p0 = 1.0, p1 = x;
-        for (int k = 2; k <= n; ++k) {
-          const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-          p0 = p1;
-          p1 = p2;
-        }
-        const double pn = n == 1 ? x : p1, pm = n == 1 ? 1.0 : p0;
-        dp = n * (x * pn - pm) / (x * x - 1.0);
-        const double dx = pn / dp;
-        x -= dx;
-        if (fabs(dx) < 1e-16) break;
-      }
-      cfg.gl_node[n - 1 - i] = x;
-      cfg.gl_weight[n - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
-    }
-  }
+  gauss_legendre(b.quad_level, cfg.gl_node, cfg.gl_weight);
   if (jac)
     hipLaunchKernelGGL(integrate_kernel<true>, dim3(kPersist / 4), dim3(256), 0, stream, b, s, cfg);
   else
     hipLaunchKernelGGL(integrate_kernel<false>, dim3(kPersist / 4), dim3(256), 0, stream, b, s, cfg);
+}
+
+// convolution of the listed tiles into `out` (and J in the Jacobian stage)
+void conv_part(const apx_batch& b, const Scratch& s, int stage, float* out, hipStream_t stream) {
+  const bool jac = stage == ST_JAC;
   const int kmax = b.max_kernel;
   const size_t tile = (size_t)8 * fast_nt(kmax) * (size_t)(64 + kmax - 1);
   const size_t lds = sizeof(float) * std::max(tile, (size_t)4 * (jac ? 3 : 1) * kRYF * 64);
@@ -826,6 +832,13 @@ int run_stage(const apx_batch& b, const Scratch& s, int stage, const double* rep
     hipLaunchKernelGGL(conv_kernel<true>, dim3(tc), dim3(256), lds, stream, b, s, out);
   else
     hipLaunchKernelGGL(conv_kernel<false>, dim3(tc), dim3(256), lds, stream, b, s, out);
+}
+
+// one render stage over the fits active in `stage`; parts: 1 planes, 2 conv
+int run_stage(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
+              float* out, hipStream_t stream, int parts = 3) {
+  if (parts & 1) render_part(b, s, stage, rep_override, stream);
+  if (parts & 2) conv_part(b, s, stage, out, stream);
   return check_launch("batch stage");
 }
 
@@ -885,12 +898,14 @@ static int check_batch(const apx_batch* b) {
   return APX_OK;
 }
 
-int batch_render(const apx_batch* b, const double* rep, float* out, hipStream_t stream) {
+int batch_render(const apx_batch* b, const double* rep, float* out, int32_t parts,
+                 hipStream_t stream) {
   int rc = check_batch(b);
   if (rc) return rc;
   if (!rep || !out) return set_error(APX_E_INVALID, "batch_render: null pointer");
+  if (parts < 1 || parts > 3) return set_error(APX_E_INVALID, "batch_render: parts must be 1, 2 or 3");
   const Scratch s = carve(*b);
-  return run_stage(*b, s, ST_ALL, rep, out, stream);
+  return run_stage(*b, s, ST_ALL, rep, out, stream, parts);
 }
 
 int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream) {

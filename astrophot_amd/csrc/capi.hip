@@ -105,8 +105,9 @@ int64_t apx_batch_state_bytes(void) { return apx::batch_state_bytes(); }
 
 int64_t apx_batch_scratch_bytes(const apx_batch* batch) { return apx::batch_scratch_bytes(batch); }
 
-int apx_batch_render(const apx_batch* batch, const double* rep, float* out, void* stream) {
-  return apx::batch_render(batch, rep, out, static_cast<hipStream_t>(stream));
+int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int32_t parts,
+                     void* stream) {
+  return apx::batch_render(batch, rep, out, parts, static_cast<hipStream_t>(stream));
 }
 
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream) {

@@ -1,16 +1,20 @@
 """Benchmark of the image-model hot path on one MI355X.
 
-Workload (N=1): one 512 x 512 image, Sersic bulge + exponential disk, 51 x 51
-Moffat PSF (beta = 3, FWHM = 3 px), threshold-selected adaptive supersampling
-of the centre (AstroPhot's defaults), PSF convolution.  One step = one
-``model.sample()`` of the whole group (render + supersample + direct
-convolution of both components).  value = rendered output Mpix/s.
+Workload (N=1): 4096 independent Sersic cutouts (side log-uniform 32-256 px,
+n ~ U(0.5, 6), Re log-uniform 2-40 px, q ~ U(0.2, 1), PA ~ U(0, pi), S/N
+log-uniform 10-1000, Gaussian PSF sigma ~ U(1, 3) in a 4-sigma box), each
+fitted by AstroPhot's Levenberg-Marquardt (psf_mode "full", threshold
+supersampling, defaults of fit/lm.py) from a +-10 % perturbed start.  One step
+= fitting all 4096 cutouts to convergence with the batched device-resident LM
+(HIP-graph ticks).  value = LM fits/s.
 
-Also reported: Jacobian evaluations/s of the same model (13 columns), a plain
-LM fit of it, the dominant kernel's roofline (live HIP-event timing on the
-launch stream), and the float64 oracle timed on the host cores (baseline).
+Beside it: the plain single-galaxy GPU path (astrophot_amd.fit.LM, one cutout
+at a time) on a subset, with the batched results checked against it (matched
+chi^2 and parameters); the batched convolution kernel's roofline measured live
+with HIP events; the float64 oracle LM on the host cores as the CPU baseline;
+and the 512x512 bulge+disk render (render Mpix/s, Jacobian evals/s).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ngal 4096] [--no-cpu]
 """
 
 from __future__ import annotations
@@ -29,19 +33,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "LM fits/s, batched+learned-init vs plain, at matched chi2; render Mpix/s"
-PEAK_FP32 = 157.3e12      # MI355X dense FP32 (TFLOP/s published figure)
-PEAK_HBM = 8.0e12         # MI355X HBM3E bytes/s
+PEAK_FP32 = 157.3e12      # MI355X dense FP32, FLOP/s
 SEED = 20261202
-SIZE = 512
-PSF_SIDE = 51
-BETA = 3.0
-FWHM = 3.0
+
+# C1: 512x512 bulge + disk, 51x51 Moffat PSF
+C1_SIZE, C1_PSF, BETA, FWHM = 512, 51, 3.0, 3.0
+
+
+def moffat_rd(beta=BETA, fwhm=FWHM):
+    return fwhm / (2 * math.sqrt(2 ** (1 / beta) - 1))
 
 
 def c1_params(seed=SEED):
-    """Seeded bulge + disk parameters (numpy PCG64)."""
     rng = np.random.default_rng(seed)
-    cx, cy = SIZE / 2 + rng.uniform(-0.5, 0.5, 2)
+    cx, cy = C1_SIZE / 2 + rng.uniform(-0.5, 0.5, 2)
     pa = rng.uniform(0, math.pi)
     bulge = {"center": [cx, cy], "q": rng.uniform(0.3, 1.0), "PA": pa, "n": rng.uniform(2, 5),
              "Re": rng.uniform(5, 20), "Ie": 1.0}
@@ -49,148 +54,209 @@ def c1_params(seed=SEED):
     return bulge, disk
 
 
-def moffat_rd(beta=BETA, fwhm=FWHM):
-    return fwhm / (2 * math.sqrt(2 ** (1 / beta) - 1))
-
-
-def build_gpu_model(psf, data=None, var=None):
+def c1_model(psf):
     import astrophot_amd as ap
 
     bulge, disk = c1_params()
-    tgt = ap.image.Target_Image(data=np.zeros((SIZE, SIZE)) if data is None else data, pixelscale=1.0,
-                                variance=var, psf=ap.image.PSF_Image(data=psf, pixelscale=1.0))
+    tgt = ap.image.Target_Image(data=np.zeros((C1_SIZE, C1_SIZE)), pixelscale=1.0,
+                                psf=ap.image.PSF_Image(data=psf, pixelscale=1.0))
     b = ap.models.AstroPhot_Model(name="bulge", model_type="sersic galaxy model", target=tgt, parameters=bulge,
                                   psf_mode="full")
-    d = ap.models.AstroPhot_Model(name="disk", model_type="exponential galaxy model", target=tgt,
-                                  parameters=disk, psf_mode="full")
+    d = ap.models.AstroPhot_Model(name="disk", model_type="exponential galaxy model", target=tgt, parameters=disk,
+                                  psf_mode="full")
     return ap.models.AstroPhot_Model(name="galaxy", model_type="group model", models=[b, d], target=tgt)
 
 
-def cpu_baseline(psf, budget_s=20.0):
-    """float64 oracle (the reference algorithm restated, FFT convolution) on the
-    host cores: forward renders of the same model until ~budget_s."""
-    from oracle import hotpath as H
-
-    ncores = len(os.sched_getaffinity(0))
-    torch.set_num_threads(ncores)
-    bulge, disk = c1_params()
-    g = H.Grid(SIZE, SIZE, 1.0)
-    pt = torch.tensor(psf, dtype=torch.float64)
-    m = H.Model([H.Component("sersic", g.copy(), bulge, psf=pt, psf_mode="full"),
-                 H.Component("exponential", g.copy(), disk, psf=pt, psf_mode="full")], window=g.copy())
-    rep = m.vector_representation()
-    m.forward(rep)  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or n < 2:
-        m.forward(rep)
-        n += 1
-    dt = (time.perf_counter() - t0) / n
-    return {"value": SIZE * SIZE / dt / 1e6, "unit": "Mpix/s", "cores": ncores, "kind": "port",
-            "sample": f"{n} float64 forward renders of the 512x512 bulge+disk model (oracle, torch CPU, FFT conv), "
-                      f"{dt:.3f} s each"}
-
-
-def main():
-    ap_ = argparse.ArgumentParser()
-    ap_.add_argument("--gpus", type=int, default=1)
-    ap_.add_argument("--steps", type=int, default=50)
-    ap_.add_argument("--warmup", type=int, default=5)
-    ap_.add_argument("--no-cpu", action="store_true")
-    ap_.add_argument("--no-fit", action="store_true")
-    args = ap_.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              f"(the fits of this workload are independent; see DESIGN.md for the sharding plan)",
-              file=sys.stderr)
-        return 2
-
-    import astrophot_amd as ap
-    from astrophot_amd import _engine
+def c1_render_leg(steps=50):
+    """render Mpix/s and Jacobian evals/s of the 512x512 bulge+disk model."""
     from astrophot_amd.utils import moffat_psf
 
-    psf = moffat_psf(BETA, moffat_rd(), PSF_SIDE, 1.0)
-    model = build_gpu_model(psf)
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
+    model = c1_model(moffat_psf(BETA, moffat_rd(), C1_PSF, 1.0))
+    for _ in range(5):
         model()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    s = torch.cuda.Event(enable_timing=True)
-    e = torch.cuda.Event(enable_timing=True)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     s.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         model()
     e.record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = s.elapsed_time(e) / args.steps
-    mpix = SIZE * SIZE / (ms * 1e-3) / 1e6
-
-    # dominant kernel: live event timing of every launch over a second timed region
-    _engine.TIMING = {}
-    for _ in range(args.steps):
-        model()
-    torch.cuda.synchronize()
-    timing, _engine.TIMING = _engine.TIMING, None
-    kern = {}
-    for name, evs in timing.items():
-        t = [a.elapsed_time(b) for a, b, _ in evs]
-        kern[name] = {"launches": len(evs), "avg_us": 1e3 * float(np.mean(t)),
-                      "work_per_launch": float(np.mean([w for _, _, w in evs]))}
-    conv = kern["convolve"]
-    achieved = conv["work_per_launch"] / (conv["avg_us"] * 1e-6) / 1e12
-    roof = {"bound": "valu", "kernel": "convolve_kernel", "achieved": achieved, "peak": PEAK_FP32 / 1e12,
-            "unit": "TFLOP/s", "frac": achieved * 1e12 / PEAK_FP32, "traffic": None,
-            "algorithmic_per_launch": conv["work_per_launch"], "avg_launch_us": conv["avg_us"]}
-
-    # Jacobian evaluations/s (13 columns, representation units)
+    ms = s.elapsed_time(e) / steps
     rep = model.parameters.vector_representation()
     for _ in range(2):
         model.jacobian(parameters=rep, as_representation=True)
     torch.cuda.synchronize()
-    nj = max(3, args.steps // 5)
+    nj = max(5, steps // 5)
     s.record()
     for _ in range(nj):
         model.jacobian(parameters=rep, as_representation=True)
     e.record()
     torch.cuda.synchronize()
-    jac_ms = s.elapsed_time(e) / nj
+    jms = s.elapsed_time(e) / nj
+    return {"workload": "512x512 Sersic bulge + exponential disk, 51x51 Moffat PSF", "render_ms": ms,
+            "render_mpix_per_s": C1_SIZE * C1_SIZE / (ms * 1e-3) / 1e6, "jacobian_ms": jms,
+            "jacobian_evals_per_s": 1e3 / jms, "jacobian_columns": int(rep.numel())}
 
-    fit = None
-    if not args.no_fit:
-        from oracle import hotpath as H  # truth image for the fit only
-        truth = model().data.double().cpu().numpy()
-        rng = np.random.default_rng(SEED + 1)
-        var = truth / 5.0 + 0.01
-        data = truth + rng.normal(size=truth.shape) * np.sqrt(var)
-        fm = build_gpu_model(psf, data=data, var=var)
-        start = fm.parameters.vector_representation().clone()
-        start = start * (1 + torch.as_tensor(rng.uniform(-0.02, 0.02, len(start))))
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        res = ap.fit.LM(fm, initial_state=start).fit()
-        torch.cuda.synchronize()
-        fit = {"fit_s": time.perf_counter() - t1, "iterations": len(res.loss_history) - 1,
-               "chi2_start": res.loss_history[0], "chi2_end": res.loss_history[-1], "message": res.message,
-               "forwards": res.n_forward, "jacobians": res.n_jacobian}
+
+def c2_problem(ngal, seed=SEED):
+    """Seeded C2 population rendered by the batch renderer, plus noise."""
+    from astrophot_amd.batch import BatchFit, Cutout, values_to_rep
+    from astrophot_amd.population import perturbed_start, sample_population
+
+    pop = sample_population(ngal, seed)
+    cuts = [Cutout(np.broadcast_to(0.0, (s, s)), None, p) for s, p in zip(pop.sides, pop.psfs)]
+    dev = "cuda"
+    npix = int(sum(int(s) * int(s) for s in pop.sides))
+    zeros = torch.zeros(npix, dtype=torch.float32, device=dev)
+    bf0 = BatchFit(cuts, data=zeros, variance=torch.ones_like(zeros))
+    truth = bf0.render(values_to_rep(pop.truth))
+    sigma = torch.repeat_interleave(torch.tensor(pop.sky_sigma, dtype=torch.float32, device=dev),
+                                    torch.tensor(pop.sides.astype(np.int64) ** 2, device=dev))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 1)
+    data = truth + sigma * torch.randn(npix, generator=gen, device=dev)
+    var = sigma.double() ** 2
+    del bf0
+    start = perturbed_start(pop.truth, seed + 2)
+    return pop, cuts, data, var, start
+
+
+def plain_fits(pop, cuts, data, var, start, idx):
+    """The single-galaxy GPU path (astrophot_amd.fit.LM) on cutouts ``idx``."""
+    import astrophot_amd as ap
+
+    offs = np.concatenate([[0], np.cumsum(pop.sides.astype(np.int64) ** 2)])
+    out = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in idx:
+        s = int(pop.sides[k])
+        d = data[offs[k]:offs[k + 1]].view(s, s)
+        v = var[offs[k]:offs[k + 1]].view(s, s)
+        tgt = ap.image.Target_Image(data=d, pixelscale=1.0, variance=v,
+                                    psf=ap.image.PSF_Image(data=cuts[k].psf, pixelscale=1.0))
+        p0 = start[k]
+        m = ap.models.AstroPhot_Model(name=f"c{k}", model_type="sersic galaxy model", target=tgt, psf_mode="full",
+                                      parameters={"center": [p0[0], p0[1]], "q": p0[2], "PA": p0[3], "n": p0[4],
+                                                  "Re": p0[5], "Ie": p0[6]})
+        lm = ap.fit.LM(m, initial_state=m.parameters.vector_representation()).fit()
+        out.append((m.parameters.vector_values().numpy(), min(lm.loss_history), lm.message))
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def cpu_baseline(pop, data, var, start, budget_s=20.0):
+    """float64 oracle LM (the reference algorithm restated, FFT convolution)
+    on the host cores, cutout after cutout until ~budget_s."""
+    from oracle import hotpath as H
+
+    ncores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(ncores)
+    offs = np.concatenate([[0], np.cumsum(pop.sides.astype(np.int64) ** 2)])
+    dh, vh = data.double().cpu(), var.double().cpu()
+    n, t0, npx = 0, time.perf_counter(), 0
+    while time.perf_counter() - t0 < budget_s and n < len(pop.sides):
+        k = n
+        s = int(pop.sides[k])
+        p0 = start[k]
+        p = {"center": [p0[0], p0[1]], "q": p0[2], "PA": p0[3], "n": p0[4], "Re": p0[5], "Ie": p0[6]}
+        om = H.Model([H.Component("sersic", H.Grid(s, s), p, psf=torch.tensor(pop.psfs[k]), psf_mode="full")])
+        H.LM(om, dh[offs[k]:offs[k + 1]], weight=1.0 / vh[offs[k]:offs[k + 1]]).fit()
+        n += 1
+        npx += s * s
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "fits/s", "cores": ncores, "kind": "port",
+            "sample": f"the first {n} cutouts of the seeded population fitted one after another by the float64 "
+                      f"oracle LM (torch CPU, FFT conv, {npx} px in total) in {dt:.1f} s; the population's mean "
+                      f"cutout is {float(np.mean(pop.sides.astype(float) ** 2)):.0f} px"}
+
+
+def main():
+    ap_ = argparse.ArgumentParser()
+    ap_.add_argument("--gpus", type=int, default=1)
+    ap_.add_argument("--steps", type=int, default=3)
+    ap_.add_argument("--warmup", type=int, default=1)
+    ap_.add_argument("--ngal", type=int, default=4096)
+    ap_.add_argument("--plain", type=int, default=24, help="cutouts fitted by the plain single-galaxy path")
+    ap_.add_argument("--no-cpu", action="store_true")
+    ap_.add_argument("--no-c1", action="store_true")
+    args = ap_.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the fits are independent "
+              f"and would shard by cutout, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from astrophot_amd.batch import BatchFit, values_to_rep
+
+    pop, cuts, data, var, start = c2_problem(args.ngal)
+    bf = BatchFit(cuts, data=data, variance=var)
+    rep0 = torch.tensor(values_to_rep(start), dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        res = bf.fit(rep0)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    s.record()
+    for _ in range(args.steps):
+        res = bf.fit(rep0)
+    e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = s.elapsed_time(e) / args.steps
+    fits_per_s = args.ngal / (ms * 1e-3)
+    status = {int(k): int(v) for k, v in zip(*np.unique(res["status"], return_counts=True))}
+
+    # dominant kernel: the batched convolution of every cutout, timed live
+    truth_rep = torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda")
+    out = torch.zeros(bf.npix, dtype=torch.float32, device="cuda")
+    bf.render(truth_rep, out=out, parts=1)
+    torch.cuda.synchronize()
+    evs = []
+    for _ in range(10):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        bf.render(truth_rep, out=out, parts=2)
+        b.record()
+        evs.append((a, b))
+    torch.cuda.synchronize()
+    conv_us = 1e3 * float(np.mean([a.elapsed_time(b) for a, b in evs[2:]]))
+    flops = bf.conv_flops()
+    achieved = flops / (conv_us * 1e-6)
+    roof = {"bound": "valu", "kernel": "conv_kernel<false> (batched value convolution, all 4096 cutouts)",
+            "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32, "traffic": None, "algorithmic_per_launch": flops,
+            "avg_launch_us": conv_us}
+
+    # plain single-galaxy GPU path on a subset, and agreement with the batch
+    idx = list(range(min(args.plain, args.ngal)))
+    plain, t_plain = plain_fits(pop, cuts, data, var, start, idx)
+    plain_fps = len(idx) / t_plain
+    perr, cerr = 0.0, 0.0
+    for k, (v, c2, msg) in zip(idx, plain):
+        perr = max(perr, float(np.max(np.abs(res["values"][k] - v) / np.maximum(np.abs(v), 1.0))))
+        cerr = max(cerr, abs(float(res["chi2"][k]) - c2) / c2)
 
     line = {
-        "metric": METRIC, "value": mpix, "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": fits_per_s, "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "C1: 512x512 Sersic bulge + exponential disk, 51x51 Moffat PSF (beta=3, FWHM=3), "
-                               "threshold-adaptive supersampling, one model.sample() per step",
-                   "image": [SIZE, SIZE], "psf": PSF_SIDE, "seed": SEED, "parallelism": "single GPU"},
+        "config": {"workload": "C2: 4096 Sersic cutouts (side 32-256 px log-uniform), per-cutout Gaussian PSF, "
+                               "batched device-resident LM from a +-10% start; one step = all cutouts fitted",
+                   "ngal": args.ngal, "total_pixels": int(bf.npix), "seed": SEED, "init": "perturbed truth +-10%",
+                   "parallelism": "single GPU"},
         "roofline": roof,
-        "kernels": kern,
-        "jacobian_evals_per_s": 1e3 / jac_ms,
-        "jacobian_ms": jac_ms,
-        "lm_fit": fit,
+        "batched": {"fits_per_s": fits_per_s, "ticks": int(res["ticks"]), "status_counts": status,
+                    "mean_iterations": float(np.mean(res["iterations"]))},
+        "plain_gpu": {"fits_per_s": plain_fps, "cutouts": len(idx), "seconds": t_plain},
+        "batched_vs_plain": fits_per_s / plain_fps,
+        "batched_vs_plain_agreement": {"max_param_rel": perr, "max_chi2_rel": cerr},
         "wall_s_timed_region": wall,
     }
+    if not args.no_c1:
+        line["c1"] = c1_render_leg()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(psf)
-        line["gpu_vs_cpu"] = mpix / line["cpu_baseline"]["value"]
+        line["cpu_baseline"] = cpu_baseline(pop, data, var, start)
+        line["gpu_vs_cpu"] = fits_per_s / line["cpu_baseline"]["value"]
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

@@ -246,8 +246,11 @@ int apx_batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* batch);
 int64_t apx_batch_state_bytes(void);
 int64_t apx_batch_scratch_bytes(const apx_batch* batch);
 
-/* Render every cutout at rep (device, ngal x 7 float64) into out (npix). */
-int apx_batch_render(const apx_batch* batch, const double* rep, float* out, void* stream);
+/* Render every cutout at rep (device, ngal x 7 float64) into out (npix).
+ * parts: 1 = render the working planes only, 2 = convolve planes rendered by
+ * a previous parts=1 call, 3 = both. */
+int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int32_t parts,
+                     void* stream);
 
 /* Start the fits from rep0 (device, ngal x 7). */
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream);

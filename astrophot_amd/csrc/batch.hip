@@ -62,37 +62,55 @@ struct GalEval {
 };
 
 struct Scratch {
-  int32_t* cnt;       // [0] fits, [1] chunks, [2] tiles, [3] selected pixels
+  int32_t* cnt;       // [0] fits, [1] chunks, [2] tiles, [3] refined pixels
   int32_t* gal_list;  // ngal
   int2* chunk_list;   // nchunk
   int2* tile_list;    // ntile
-  int2* sel_list;     // nwork
+  int32_t* gal_cnt;   // ngal: pixels each fit sends to the level-2 quadrature
+  int32_t* gal_off;   // ngal + 1: exclusive prefix of gal_cnt
+  int32_t* deep;      // nwork: per-fit segments (at work_off) of those pixels
+  float* mid;         // nwork: midpoint samples
+  double* part;       // ntile * kPart: per-tile partial sums of the reductions
   GalEval* ev;        // ngal
 };
 
+constexpr int kPart = 64;  // doubles per tile partial (49 H + 7 g + 1 chi2 fit the largest)
+
 __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
-__host__ inline Scratch carve(const apx_batch& b) {
+__host__ inline int64_t scratch_layout(const apx_batch& b, Scratch* s) {
   char* p = static_cast<char*>(b.scratch);
-  Scratch s;
   int64_t off = 0;
-  s.cnt = reinterpret_cast<int32_t*>(p + off);
-  off += 256;
-  s.gal_list = reinterpret_cast<int32_t*>(p + off);
-  off += align256(4 * (int64_t)b.ngal);
-  s.chunk_list = reinterpret_cast<int2*>(p + off);
-  off += align256(8 * b.nchunk);
-  s.tile_list = reinterpret_cast<int2*>(p + off);
-  off += align256(8 * b.ntile);
-  s.sel_list = reinterpret_cast<int2*>(p + off);
-  off += align256(8 * b.nwork);
-  s.ev = reinterpret_cast<GalEval*>(p + off);
+  auto take = [&](int64_t bytes) {
+    char* q = p ? p + off : nullptr;
+    off += align256(bytes);
+    return q;
+  };
+  Scratch t;
+  t.cnt = reinterpret_cast<int32_t*>(take(256));
+  t.gal_list = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
+  t.chunk_list = reinterpret_cast<int2*>(take(8 * b.nchunk));
+  t.tile_list = reinterpret_cast<int2*>(take(8 * b.ntile));
+  t.gal_cnt = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
+  t.gal_off = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
+  t.deep = reinterpret_cast<int32_t*>(take(4 * b.nwork));
+  t.mid = reinterpret_cast<float*>(take(4 * b.nwork));
+  t.part = reinterpret_cast<double*>(take(8 * kPart * b.ntile));
+  t.ev = reinterpret_cast<GalEval*>(take((int64_t)sizeof(GalEval) * b.ngal));
+  if (s) *s = t;
+  return off;
+}
+
+__host__ inline Scratch carve(const apx_batch& b) {
+  Scratch s;
+  scratch_layout(b, &s);
   return s;
 }
 
 __host__ int64_t scratch_bytes(const apx_batch& b) {
-  return 256 + align256(4 * (int64_t)b.ngal) + align256(8 * b.nchunk) + align256(8 * b.ntile) +
-         align256(8 * b.nwork) + align256((int64_t)sizeof(GalEval) * b.ngal);
+  apx_batch c = b;
+  c.scratch = nullptr;
+  return scratch_layout(c, nullptr);
 }
 
 __device__ __forceinline__ GalState* st_of(const apx_batch& b, int g) {
@@ -114,6 +132,7 @@ __global__ void reset_kernel(int32_t* cnt) {
 __global__ __launch_bounds__(256) void build_kernel(apx_batch b, Scratch s, int stage) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= b.ngal) return;
+  s.gal_cnt[g] = 0;
   const int ph = st_of(b, g)->phase;
   const bool active = stage == ST_ALL || (stage == ST_JAC ? (ph == PH_INIT || ph == PH_NEEDJ)
                                                          : ph == PH_SUB);
@@ -233,6 +252,72 @@ __global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, in
 
 // ---------------------------------------------------------------------------
 // render stages
+//
+// Pixel selection and adaptive quadrature (utils/operations.py:150-247, driven
+// by models/_model_methods.py:82-184).  Samples are evaluated in float32 and
+// summed in float64; every threshold decision whose float32 value lies within
+// kBandRel of the threshold is taken again from float64 samples, so the set of
+// refined pixels and sub-pixels is the one float64 arithmetic gives.
+constexpr double kBandRel = 1e-4;
+
+// Gauss-Legendre table of order n and the sub-gridding g
+struct QuadTab {
+  int n, nn, g, gg, max_depth;
+  float node[APX_MAX_QUAD_LEVEL];
+  float w2[APX_MAX_QUAD_LEVEL * APX_MAX_QUAD_LEVEL];  // w_a w_b / 4, node k = (k % n, k / n)
+  double noded[APX_MAX_QUAD_LEVEL];
+  double w2d[APX_MAX_QUAD_LEVEL * APX_MAX_QUAD_LEVEL];
+};
+
+// n x n Gauss-Legendre sample of the square of half-width h (in pixel-matrix
+// units) centred on (X, Y) (operations.py:123-147); `centre` is the middle node
+template <bool GRAD>
+__device__ __forceinline__ float quad_f(const Prof<float>& p, const QuadTab& t, float X, float Y,
+                                        float h, float scale, float* g, float& centre) {
+  float res = 0.0f, gl[APX_NSLOT];
+  if (GRAD) {
+#pragma unroll
+    for (int q = 0; q < APX_NSLOT; ++q) g[q] = 0.0f;
+  }
+  const float h00 = h * p.P00, h01 = h * p.P01, h10 = h * p.P10, h11 = h * p.P11;
+  for (int k = 0; k < t.nn; ++k) {
+    const float u = t.node[k % t.n], v = t.node[k / t.n], w = t.w2[k];
+    const float val =
+        eval_profile<float, GRAD>(p, X + h00 * u + h01 * v, Y + h10 * u + h11 * v, scale, gl);
+    res = fmaf(w, val, res);
+    if (k == t.nn / 2) centre = val;
+    if (GRAD) {
+#pragma unroll
+      for (int q = 0; q < APX_NSLOT; ++q) g[q] = fmaf(w, gl[q], g[q]);
+    }
+  }
+  return res;
+}
+
+__device__ __forceinline__ double quad_d(const Prof<double>& p, const QuadTab& t, double X,
+                                         double Y, double h, double scale, double& centre) {
+  double res = 0.0;
+  const double h00 = h * p.P00, h01 = h * p.P01, h10 = h * p.P10, h11 = h * p.P11;
+  for (int k = 0; k < t.nn; ++k) {
+    const double u = t.noded[k % t.n], v = t.noded[k / t.n];
+    const double val = eval_profile<double, false>(p, X + h00 * u + h01 * v,
+                                                   Y + h10 * u + h11 * v, scale, nullptr);
+    res += t.w2d[k] * val;
+    if (k == t.nn / 2) centre = val;
+  }
+  return res;
+}
+
+// |res - centre| > thr, decided in float64 when float32 is too close to call
+__device__ __forceinline__ int band_test(float res, float centre, double thr) {
+  const double D = fabs((double)res - (double)centre);
+  const double band = kBandRel * (fabs((double)res) + fabs((double)centre)) + 1e-7 * thr;
+  if (D > thr + band) return 1;
+  if (D < thr - band) return 0;
+  return -1;
+}
+
+// midpoint samples of every working pixel (and their parameter partials)
 template <bool GRAD>
 __global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s) {
   for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
@@ -244,31 +329,37 @@ __global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s) {
     float X, Y, gr[APX_NSLOT];
     pixel_xy(E.pf, E.ci, E.cj, pix % E.nxw, pix / E.nxw, X, Y);
     const float val = eval_profile<float, GRAD>(E.pf, X, Y, 1.0f, gr);
-    float* pl = b.planes + G.work_off + pix;
-    pl[0] = val;
+    s.mid[G.work_off + pix] = val;
     if (GRAD) {
       // planes 1..5: q, PA, n, Re, Ie in representation units
+      float* pl = b.planes + G.work_off + pix;
 #pragma unroll
       for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = gr[c + 1] * (float)E.dvdr[c + 1];
     }
   }
 }
 
-__global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s) {
+// curvature selection (_model_methods.py:87-97, 155-166) and the level-1
+// quadrature of each selected pixel, one lane per pixel; pixels the level-1
+// test sends deeper go to their fit's segment of the `deep` list
+template <bool GRAD>
+__global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, QuadTab t) {
+  const int lane = threadIdx.x & 63;
   for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
     const int2 e = s.chunk_list[k];
     const apx_batch_galaxy G = b.gal[e.x];
     const GalEval E = s.ev[e.x];
     const int pix = e.y * kChunk + threadIdx.x;
-    bool sel = false;
+    bool deeper = false;
     if (pix < E.nxw * E.nyw) {
       const int i = pix % E.nxw, j = pix / E.nxw;
       const int ii = min(max(i, 1), E.nxw - 2), jj = min(max(j, 1), E.nyw - 2);
-      const float* v = b.planes + G.work_off + (int64_t)jj * E.nxw + ii;
+      const float* v = s.mid + G.work_off + (int64_t)jj * E.nxw + ii;
       const float l = v[-1], r = v[1], u = v[-E.nxw], d = v[E.nxw], m = v[0];
       const float metric = fabsf(l + r + u + d - 4.0f * m);
       const float scale = fabsf(l) + fabsf(r) + fabsf(u) + fabsf(d) + 4.0f * fabsf(m);
       const double band = 1.6e-5 * (double)scale + 2.0e-7 * E.thr;
+      bool sel = false;
       if ((double)metric > E.thr + band) {
         sel = true;
       } else if ((double)metric >= E.thr - band) {  // float64 re-decision
@@ -282,51 +373,230 @@ __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s) {
         const double mid = (i == ii && j == jj) ? mc : at(i, j);
         sel = fabs(mid - (mid + curv)) > E.thr;
       }
+      float* pl = b.planes + G.work_off + pix;
+      if (!sel) {
+        pl[0] = s.mid[G.work_off + pix];
+      } else {
+        float X0, Y0, gs[APX_NSLOT], centre = 0.0f;
+        pixel_xy(E.pf, E.ci, E.cj, i, j, X0, Y0);
+        const float r1 = quad_f<GRAD>(E.pf, t, X0, Y0, 0.5f, 1.0f, gs, centre);
+        if (t.max_depth > 1) {
+          const int dec = band_test(r1, centre, E.thr);
+          if (dec < 0) {
+            double X, Y, cd = 0.0;
+            pixel_xy(E.pd, E.ci, E.cj, i, j, X, Y);
+            const double rd = quad_d(E.pd, t, X, Y, 0.5, 1.0, cd);
+            deeper = fabs(rd - cd) > E.thr;
+          } else {
+            deeper = dec == 1;
+          }
+        }
+        if (!deeper) {
+          pl[0] = r1;
+          if (GRAD) {
+#pragma unroll
+            for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = gs[c + 1] * (float)E.dvdr[c + 1];
+          }
+        }
+      }
     }
-    const uint64_t ballot = __ballot(sel);
-    if (ballot) {
-      const int lane = threadIdx.x & 63;
+    const uint64_t ballot = __ballot(deeper);
+    if (ballot) {  // one atomic per wave on the fit's own counter
       int base = 0;
-      if (lane == 0) base = atomicAdd(&s.cnt[3], (int)__popcll(ballot));
+      if (lane == 0) base = atomicAdd(&s.gal_cnt[e.x], (int)__popcll(ballot));
       base = __shfl(base, 0, 64);
-      if (sel) {
+      if (deeper) {
         const uint64_t below = ballot & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-        s.sel_list[base + (int)__popcll(below)] = make_int2(e.x, pix);
+        s.deep[G.work_off + base + (int)__popcll(below)] = pix;
       }
     }
   }
 }
 
-template <bool GRAD>
-__global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, apx_component cfg) {
-  const int lane = threadIdx.x & 63;
-  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int nwaves = gridDim.x * 4;
-  for (int k = wave; k < s.cnt[3]; k += nwaves) {
-    const int2 e = s.sel_list[k];
-    const apx_batch_galaxy G = b.gal[e.x];
-    const GalEval E = s.ev[e.x];
-    double out[1 + APX_NSLOT];
-    integrate_pixel<GRAD>(cfg, E.pd, E.nxw, E.ci, E.cj, e.y, E.thr, lane, out);
-    if (lane == 0) {
-      float* pl = b.planes + G.work_off + e.y;
-      pl[0] = (float)out[0];
+// exclusive prefix of the per-fit counts: one workgroup of 1024
+__global__ __launch_bounds__(1024) void scan_kernel(apx_batch b, Scratch s) {
+  __shared__ int32_t wsums[16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < b.ngal; base += 1024) {
+    const int g = base + threadIdx.x;
+    const int v = g < b.ngal ? s.gal_cnt[g] : 0;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsums[wv] = x;
+    __syncthreads();
+    int pre = 0, total = 0;
+    for (int w = 0; w < 16; ++w) {
+      pre += w < wv ? wsums[w] : 0;
+      total += wsums[w];
+    }
+    if (g < b.ngal) s.gal_off[g] = carry + pre + x - v;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    s.gal_off[b.ngal] = carry;
+    s.cnt[3] = carry;
+  }
+}
+
+// levels 2 and 3 of the quadrature: a group of GS lanes per listed pixel, one
+// lane per level-2 sub-pixel; each sub-pixel the level-2 test flags is then
+// integrated by the whole group (operations.py:229-247)
+template <bool GRAD, int GS>
+__global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, QuadTab t) {
+  constexpr int NG = 64 / GS;
+  const int lane = threadIdx.x & 63, gl = lane % GS, grp = lane / GS;
+  const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
+  const int total = s.cnt[3];
+  const int g_ = t.g, gg = t.gg, nq = gg * t.nn;
+  const float ig = 1.0f / (float)g_;
+  const float sub0 = -(float)(g_ - 1) / (2.0f * (float)g_), subd = ig;
+  const double igd = 1.0 / (double)g_;
+  const double sub0d = -(double)(g_ - 1) / (2.0 * g_);
+  for (int k0 = wave * NG; k0 < total; k0 += nwaves * NG) {
+    const int k = k0 + grp;
+    const bool active = k < total;
+    int gal = 0, pix = 0;
+    if (active) {
+      int lo = 0, hi = b.ngal;  // gal_off[lo] <= k < gal_off[hi]
+      while (hi - lo > 1) {
+        const int md = (lo + hi) >> 1;
+        if (s.gal_off[md] <= k) lo = md;
+        else hi = md;
+      }
+      gal = lo;
+      pix = s.deep[b.gal[gal].work_off + (k - s.gal_off[gal])];
+    }
+    const GalEval* E = s.ev + gal;  // fit 0's constants for an idle group: read, never used
+    const Prof<float> pf = E->pf;
+    const int nxw = active ? E->nxw : 1, ci = E->ci, cj = E->cj;
+    const int pi = pix % nxw, pj = pix / nxw;
+    float X0, Y0;
+    pixel_xy(pf, ci, cj, pi, pj, X0, Y0);
+    double acc[1 + APX_NSLOT];
+#pragma unroll
+    for (int q = 0; q <= APX_NSLOT; ++q) acc[q] = 0.0;
+    // level 2
+    bool deeper = false;
+    float x2 = 0.0f, y2 = 0.0f;
+    if (active && gl < gg) {
+      const float a = sub0 + subd * (float)(gl % g_), c = sub0 + subd * (float)(gl / g_);
+      x2 = X0 + pf.P00 * a + pf.P01 * c;
+      y2 = Y0 + pf.P10 * a + pf.P11 * c;
+      float gs[APX_NSLOT], c2 = 0.0f;
+      const float r2 = quad_f<GRAD>(pf, t, x2, y2, 0.5f * ig, ig * ig, gs, c2);
+      if (t.max_depth > 2) {
+        const double thr2 = E->thr * (double)gg;
+        const int dec = band_test(r2, c2, thr2);
+        if (dec < 0) {
+          double X, Y, cd = 0.0;
+          pixel_xy(E->pd, ci, cj, pi, pj, X, Y);
+          const double ad = sub0d + igd * (gl % g_), bd = sub0d + igd * (gl / g_);
+          const Prof<double>& pd = E->pd;
+          const double rd = quad_d(pd, t, X + pd.P00 * ad + pd.P01 * bd, Y + pd.P10 * ad + pd.P11 * bd,
+                                   0.5 * igd, igd * igd, cd);
+          deeper = fabs(rd - cd) > thr2;
+        } else {
+          deeper = dec == 1;
+        }
+      }
+      if (!deeper) {
+        acc[0] = (double)r2;
+        if (GRAD) {
+#pragma unroll
+          for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] = (double)gs[q];
+        }
+      }
+    }
+    // level 3: the group integrates each flagged sub-pixel in turn
+    const uint64_t bal = __ballot(deeper);
+    uint64_t todo = GS == 64 ? bal : ((bal >> (grp * GS)) & ((1ull << GS) - 1ull));
+    const float sc3 = ig * ig * ig * ig, h3 = 0.5f * ig * ig;
+    while (__ballot(todo != 0ull)) {
+      int sp = -1;
+      if (todo) {
+        sp = __ffsll((unsigned long long)todo) - 1;
+        todo &= todo - 1ull;
+      }
+      const int src = grp * GS + (sp < 0 ? 0 : sp);
+      const float xs = __shfl(x2, src, 64), ys = __shfl(y2, src, 64);
+      if (sp >= 0) {
+        for (int e = gl; e < nq; e += GS) {
+          const int tt = e / t.nn, kk = e % t.nn;
+          const float a = (sub0 + subd * (float)(tt % g_)) * ig, c = (sub0 + subd * (float)(tt / g_)) * ig;
+          const float u = a + h3 * t.node[kk % t.n], v = c + h3 * t.node[kk / t.n];
+          float gv[APX_NSLOT];
+          const float val = eval_profile<float, GRAD>(pf, xs + pf.P00 * u + pf.P01 * v,
+                                                      ys + pf.P10 * u + pf.P11 * v, sc3, gv);
+          const double w = (double)t.w2[kk];
+          acc[0] += w * (double)val;
+          if (GRAD) {
+#pragma unroll
+            for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] += w * (double)gv[q];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q <= APX_NSLOT; ++q) {
+      if (!GRAD && q > 0) break;
+#pragma unroll
+      for (int o = GS / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+    }
+    if (active && gl == 0) {
+      float* pl = b.planes + b.gal[gal].work_off + pix;
+      pl[0] = (float)acc[0];
       if (GRAD) {
-        for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = (float)(out[1 + c + 1] * E.dvdr[c + 1]);
+#pragma unroll
+        for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = (float)(acc[1 + c + 1] * E->dvdr[c + 1]);
       }
     }
   }
 }
 
-// convolution of every listed tile; JAC: value -> out, centre derivatives ->
-// J cols 0, 1, partial planes -> J cols 2..6
-template <bool JAC>
+// ---------------------------------------------------------------------------
+// convolution of every listed tile, with the per-fit reductions of the stage
+// folded into the epilogue as per-tile partial sums:
+//   EPI_OUT     value -> out
+//   EPI_NORMAL  value -> out (model0), centre derivatives -> J cols 0, 1,
+//               partial planes -> J cols 2..6; J^T W J, J^T W (Y - m0),
+//               chi2(m0) of the tile                        (lm.py:256-260)
+//   EPI_RPP     J^T (2 / d) (W (m(x + d h) - m0) / d - W J h), d = 0.1
+//                                                           (lm.py:276-280)
+//   EPI_CHI2    sum W (Y - m)^2 of the tile                 (lm.py:290-292)
+enum : int { EPI_OUT = 0, EPI_NORMAL = 1, EPI_RPP = 2, EPI_CHI2 = 3 };
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline int conv_stage_floats(int kmax, bool jac) {
+  const int tile = 8 * fast_nt(kmax) * (64 + kmax - 1);
+  const int red = 4 * (jac ? 3 : 1) * kRYF * 64;
+  return tile > red ? tile : red;
+}
+
+// dynamic LDS of conv_kernel: staging / reduction area, plus the J and model
+// tile of the normal-equation epilogue
+__host__ inline size_t conv_lds_bytes(int kmax, bool jac) {
+  return sizeof(float) * ((size_t)conv_stage_floats(kmax, jac) + (jac ? (NP + 1) * kRYF * 64 : 0));
+}
+constexpr size_t kConvStaticLds = sizeof(double) * 4 * (4 * 64 + 24);
+
+template <int MODE>
 __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float* out) {
   extern __shared__ float smem[];
+  __shared__ double redd[4][4 * 64 + 24];  // kConvStaticLds
+  constexpr bool JAC = MODE == EPI_NORMAL;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wu = __builtin_amdgcn_readfirstlane(w);
   constexpr int NKmax = JAC ? 3 : 1;
   constexpr int nchan = JAC ? kNChan : 1;
+  float* tJ = smem + conv_stage_floats(b.max_kernel, JAC);  // [NP][kRYF][64]
+  float* tM = tJ + NP * kRYF * 64;                          // [kRYF][64]
   for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
     const int2 e = s.tile_list[k];
     const int g = __builtin_amdgcn_readfirstlane(e.x);
@@ -336,14 +606,24 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
     const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kRYF;
     const int kh = G.ph + 2, kw = G.pw + 2;
     const int nT = fast_nt(kh), kts = fast_kts(kh), rows = 8 * nT, LW = 64 + kw - 1;
+    const int64_t tile = G.tile_off + e.y;
     // output (ox, oy) sits at working (bx + ox, by + oy); taps span +-(k-1)/2 = border
-    const int gx0 = ox0, gy0 = oy0;
     const float* kt = b.taps + G.tap_off;
+    const int ox = ox0 + lane;
+    double racc[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) racc[i] = 0.0;
+    double hv[NP];
+    if (MODE == EPI_RPP) {
+      const GalState* st = st_of(b, g);
+#pragma unroll
+      for (int i = 0; i < NP; ++i) hv[i] = st->h[i];
+    }
     for (int c = 0; c < nchan; ++c) {
       const float* plane = b.planes + (int64_t)c * b.nwork + G.work_off;
       __syncthreads();
       for (int q = threadIdx.x; q < rows * LW; q += 256) {
-        const int gy = gy0 + q / LW, gx = gx0 + q % LW;
+        const int gy = oy0 + q / LW, gx = ox0 + q % LW;
         smem[q] = (gy < nyw && gx < nxw) ? plane[(int64_t)gy * nxw + gx] : 0.0f;
       }
       __syncthreads();
@@ -362,156 +642,128 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
 #pragma unroll
           for (int r = 0; r < kRYF; ++r) red[((w * NKmax + kk) * kRYF + r) * 64 + lane] = acc[kk][r];
       __syncthreads();
-      const int ox = ox0 + lane;
       for (int q = w; q < nk * kRYF; q += 4) {
         const int kk = q / kRYF, r = q % kRYF;
         float sum = 0.0f;
 #pragma unroll
         for (int ww = 0; ww < 4; ++ww) sum += red[((ww * NKmax + kk) * kRYF + r) * 64 + lane];
         const int oy = oy0 + r;
-        if (ox >= G.nx || oy >= G.ny) continue;
+        const bool valid = ox < G.nx && oy < G.ny;
+        const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
+        if (JAC) {
+          if (col < 0) tM[r * 64 + lane] = valid ? sum : 0.0f;
+          else tJ[(col * kRYF + r) * 64 + lane] = valid ? sum : 0.0f;
+        }
+        if (!valid) continue;
         const int64_t p = G.pix_off + (int64_t)oy * G.nx + ox;
-        if (c == 0 && kk == 0)
-          out[p] = sum;
-        else if (JAC)
-          b.jac[p * NP + (c == 0 ? kk - 1 : c + 1)] = sum;
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// per-fit reductions (one workgroup per listed fit)
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
-  __shared__ double redH[4][256];
-  __shared__ double redg[4][16];
-  __shared__ double redc[4];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = lane & 15, kq = lane >> 4;
-  for (int k = blockIdx.x; k < s.cnt[0]; k += gridDim.x) {
-    const int g = s.gal_list[k];
-    const apx_batch_galaxy G = b.gal[g];
-    const int64_t n = (int64_t)G.nx * G.ny;
-    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    double accd[4] = {0.0, 0.0, 0.0, 0.0};
-    double gacc = 0.0, cacc = 0.0;
-    float gpart = 0.0f;
-    int since = 0;
-    for (int64_t base = wv * 4; base < n; base += 16) {
-      const int64_t p = base + kq;
-      float a = 0.0f, wt = 0.0f, res = 0.0f;
-      if (p < n) {
-        const int64_t q = G.pix_off + p;
-        wt = b.weight[q];
-        res = b.data[q] - b.model0[q];
-        if (r < NP) a = b.jac[q * NP + r];
-        if (r == 0) {
-          const double d = (double)b.data[q] - (double)b.model0[q];
-          cacc += (double)wt * d * d;
+        if (MODE == EPI_OUT || JAC) {
+          if (col < 0) out[p] = sum;
+          else if (JAC) b.jac[p * NP + col] = sum;
+        } else if (MODE == EPI_RPP) {
+          constexpr double d = 0.1;
+          float jr[NP];
+          double jh = 0.0;
+#pragma unroll
+          for (int i = 0; i < NP; ++i) {
+            jr[i] = b.jac[p * NP + i];
+            jh += (double)jr[i] * hv[i];
+          }
+          const double wt = (double)b.weight[p];
+          const double v = (2.0 / d) * (wt * ((double)sum - (double)b.model0[p]) / d - wt * jh);
+#pragma unroll
+          for (int i = 0; i < NP; ++i) racc[i] += (double)jr[i] * v;
+        } else {  // EPI_CHI2
+          const double dd = (double)b.data[p] - (double)sum;
+          racc[0] += (double)b.weight[p] * dd * dd;
         }
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wt * a, acc, 0, 0, 0);
-      gpart = fmaf(a * wt, res, gpart);
-      if (++since == 64) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          accd[i] += (double)acc[i];
-          acc[i] = 0.0f;
+    }
+    // ---- epilogue: the tile's partial sums
+    double* part = s.part + tile * kPart;
+    if (MODE == EPI_NORMAL) {
+      __syncthreads();  // tJ, tM complete
+      const int r16 = lane & 15, kq = lane >> 4;
+      v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+      float gpart = 0.0f;
+      double cacc = 0.0;
+      for (int stp = 0; stp < 32; ++stp) {
+        const int pp = w * 128 + stp * 4 + kq;  // tile pixel
+        const int r = pp >> 6, lx = pp & 63;
+        const int px = ox0 + lx, py = oy0 + r;
+        float a = 0.0f, wt = 0.0f, res = 0.0f;
+        if (px < G.nx && py < G.ny) {
+          const int64_t p = G.pix_off + (int64_t)py * G.nx + px;
+          const float m = tM[r * 64 + lx];
+          wt = b.weight[p];
+          res = b.data[p] - m;
+          if (r16 < NP) a = tJ[(r16 * kRYF + r) * 64 + lx];
+          if (r16 == 0) {
+            const double dd = (double)b.data[p] - (double)m;
+            cacc += (double)wt * dd * dd;
+          }
         }
-        gacc += (double)gpart;
-        gpart = 0.0f;
-        since = 0;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wt * a, acc, 0, 0, 0);
+        gpart = fmaf(a * wt, res, gpart);
       }
-    }
+      // D[4 kq + i][r16] = acc[i]
 #pragma unroll
-    for (int i = 0; i < 4; ++i) accd[i] += (double)acc[i];
-    gacc += (double)gpart;
+      for (int i = 0; i < 4; ++i) redd[w][(4 * kq + i) * 16 + r16] = (double)acc[i];
+      double gd = (double)gpart;
+      gd += __shfl_xor(gd, 16, 64);
+      gd += __shfl_xor(gd, 32, 64);
+      if (lane < 16) redd[w][256 + lane] = gd;
+      cacc = wsum(cacc);
+      if (lane == 0) redd[w][256 + 16] = cacc;
+      __syncthreads();
+      if (threadIdx.x < NP * NP + NP + 1) {
+        const int t = threadIdx.x;
+        const int idx = t < NP * NP ? (t / NP) * 16 + t % NP : (t < NP * NP + NP ? 256 + t - NP * NP : 256 + 16);
+        part[t] = redd[0][idx] + redd[1][idx] + redd[2][idx] + redd[3][idx];
+      }
+      __syncthreads();
+    } else if (MODE == EPI_RPP || MODE == EPI_CHI2) {
+      constexpr int W = MODE == EPI_RPP ? NP : 1;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) redH[wv][(4 * kq + i) * 16 + r] = accd[i];
-    gacc += __shfl_xor(gacc, 16, 64);
-    gacc += __shfl_xor(gacc, 32, 64);
-    if (lane < 16) redg[wv][lane] = gacc;
-    cacc = wsum(cacc);
-    if (lane == 0) redc[wv] = cacc;
-    __syncthreads();
-    GalState* st = st_of(b, g);
-    if (threadIdx.x < NP * NP) {
-      const int i = threadIdx.x / NP, j = threadIdx.x % NP;
-      const int e = i * 16 + j;
-      st->H[threadIdx.x] = redH[0][e] + redH[1][e] + redH[2][e] + redH[3][e];
-    } else if (threadIdx.x < NP * NP + NP) {
-      const int i = threadIdx.x - NP * NP;
-      st->g[i] = redg[0][i] + redg[1][i] + redg[2][i] + redg[3][i];
-    } else if (threadIdx.x == NP * NP + NP) {
-      st->chi2_m0 = redc[0] + redc[1] + redc[2] + redc[3];
+      for (int i = 0; i < W; ++i) {
+        const double v = wsum(racc[i]);
+        if (lane == 0) redd[w][i] = v;
+      }
+      __syncthreads();
+      if (threadIdx.x < W) {
+        const int i = threadIdx.x;
+        part[i] = redd[0][i] + redd[1][i] + redd[2][i] + redd[3][i];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void rpp_kernel(apx_batch b, Scratch s) {
-  __shared__ double red[4][8];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// per-fit sums of the tile partials (fixed order), into the LM state
+__global__ __launch_bounds__(256) void tile_reduce_kernel(apx_batch b, Scratch s, int mode) {
+  __shared__ double red[4][64];
+  const int e = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int W = mode == EPI_NORMAL ? NP * NP + NP + 1 : (mode == EPI_RPP ? NP : 1);
   for (int k = blockIdx.x; k < s.cnt[0]; k += gridDim.x) {
     const int g = s.gal_list[k];
     const apx_batch_galaxy G = b.gal[g];
-    GalState* st = st_of(b, g);
-    double h[NP], acc[NP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      h[i] = st->h[i];
-      acc[i] = 0.0;
-    }
-    const int64_t n = (int64_t)G.nx * G.ny;
-    const double d = 0.1;
-    for (int64_t p = threadIdx.x; p < n; p += 256) {
-      const int64_t q = G.pix_off + p;
-      float jr[NP];
-      double jh = 0.0;
-#pragma unroll
-      for (int i = 0; i < NP; ++i) {
-        jr[i] = b.jac[q * NP + i];
-        jh += (double)jr[i] * h[i];
-      }
-      const double wt = (double)b.weight[q];
-      const double v = (2.0 / d) * (wt * ((double)b.model1[q] - (double)b.model0[q]) / d - wt * jh);
-#pragma unroll
-      for (int i = 0; i < NP; ++i) acc[i] += (double)jr[i] * v;
-    }
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const double t = wsum(acc[i]);
-      if (lane == 0) red[wv][i] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < NP)
-      st->rpp[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
-                             red[3][threadIdx.x];
-    __syncthreads();
-  }
-}
-
-__global__ __launch_bounds__(256) void chi2_kernel(apx_batch b, Scratch s) {
-  __shared__ double red[4];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int k = blockIdx.x; k < s.cnt[0]; k += gridDim.x) {
-    const int g = s.gal_list[k];
-    const apx_batch_galaxy G = b.gal[g];
-    const int64_t n = (int64_t)G.nx * G.ny;
+    const int ntl = ((G.nx + 63) / 64) * ((G.ny + kRYF - 1) / kRYF);
     double acc = 0.0;
-    for (int64_t p = threadIdx.x; p < n; p += 256) {
-      const int64_t q = G.pix_off + p;
-      const double dd = (double)b.data[q] - (double)b.model1[q];
-      acc += (double)b.weight[q] * dd * dd;
-    }
-    acc = wsum(acc);
-    if (lane == 0) red[wv] = acc;
+    if (e < W)
+      for (int t = grp; t < ntl; t += 4) acc += s.part[(G.tile_off + t) * kPart + e];
+    red[grp][e] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < W) {
+      const double v = red[0][e] + red[1][e] + red[2][e] + red[3][e];
       GalState* st = st_of(b, g);
-      st->chi2 = (red[0] + red[1] + red[2] + red[3]) / st->ndf;
+      if (mode == EPI_NORMAL) {
+        if (e < NP * NP) st->H[e] = v;
+        else if (e < NP * NP + NP) st->g[e - NP * NP] = v;
+        else st->chi2_m0 = v;
+      } else if (mode == EPI_RPP) {
+        st->rpp[e] = v;
+      } else {
+        st->chi2 = v / st->ndf;
+      }
     }
     __syncthreads();
   }
@@ -795,6 +1047,26 @@ void gauss_legendre(int n, double* node, double* weight) {
   }
 }
 
+QuadTab quad_table(const apx_batch& b) {
+  QuadTab t = {};
+  t.n = b.quad_level;
+  t.nn = t.n * t.n;
+  t.g = b.gridding;
+  t.gg = t.g * t.g;
+  t.max_depth = b.max_depth;
+  double node[APX_MAX_QUAD_LEVEL], weight[APX_MAX_QUAD_LEVEL];
+  gauss_legendre(t.n, node, weight);
+  for (int i = 0; i < t.n; ++i) {
+    t.noded[i] = node[i];
+    t.node[i] = (float)node[i];
+  }
+  for (int k = 0; k < t.nn; ++k) {
+    t.w2d[k] = weight[k % t.n] * weight[k / t.n] * 0.25;
+    t.w2[k] = (float)t.w2d[k];
+  }
+  return t;
+}
+
 // work lists + render planes of the fits active in `stage`
 void render_part(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
                  hipStream_t stream) {
@@ -805,40 +1077,54 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
   hipLaunchKernelGGL(prepare_kernel, dim3(std::min(b.ngal, 1024)), dim3(256), 0, stream, b, s, stage,
                      rep_override);
   const int pc = (int)std::min<int64_t>(b.nchunk, 4 * kPersist);
-  if (jac)
+  const QuadTab t = quad_table(b);
+  if (jac) {
     hipLaunchKernelGGL(init_kernel<true>, dim3(pc), dim3(kChunk), 0, stream, b, s);
-  else
+    hipLaunchKernelGGL(select_kernel<true>, dim3(pc), dim3(256), 0, stream, b, s, t);
+  } else {
     hipLaunchKernelGGL(init_kernel<false>, dim3(pc), dim3(kChunk), 0, stream, b, s);
-  hipLaunchKernelGGL(select_kernel, dim3(pc), dim3(256), 0, stream, b, s);
-  apx_component cfg = {};
-  cfg.quad_level = b.quad_level;
-  cfg.gridding = b.gridding;
-  cfg.max_depth = b.max_depth;
-  gauss_legendre(b.quad_level, cfg.gl_node, cfg.gl_weight);
-  if (jac)
-    hipLaunchKernelGGL(integrate_kernel<true>, dim3(kPersist / 4), dim3(256), 0, stream, b, s, cfg);
-  else
-    hipLaunchKernelGGL(integrate_kernel<false>, dim3(kPersist / 4), dim3(256), 0, stream, b, s, cfg);
+    hipLaunchKernelGGL(select_kernel<false>, dim3(pc), dim3(256), 0, stream, b, s, t);
+  }
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b, s);
+  const dim3 ig(kPersist / 2);
+  if (t.gg <= 32) {
+    if (jac) hipLaunchKernelGGL((integrate_kernel<true, 32>), ig, dim3(256), 0, stream, b, s, t);
+    else hipLaunchKernelGGL((integrate_kernel<false, 32>), ig, dim3(256), 0, stream, b, s, t);
+  } else {
+    if (jac) hipLaunchKernelGGL((integrate_kernel<true, 64>), ig, dim3(256), 0, stream, b, s, t);
+    else hipLaunchKernelGGL((integrate_kernel<false, 64>), ig, dim3(256), 0, stream, b, s, t);
+  }
 }
 
-// convolution of the listed tiles into `out` (and J in the Jacobian stage)
-void conv_part(const apx_batch& b, const Scratch& s, int stage, float* out, hipStream_t stream) {
-  const bool jac = stage == ST_JAC;
-  const int kmax = b.max_kernel;
-  const size_t tile = (size_t)8 * fast_nt(kmax) * (size_t)(64 + kmax - 1);
-  const size_t lds = sizeof(float) * std::max(tile, (size_t)4 * (jac ? 3 : 1) * kRYF * 64);
+// convolution of the listed tiles (epilogue `mode`), then for the LM stages
+// the per-fit sums of the tile partials
+void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream) {
+  const bool jac = mode == EPI_NORMAL;
+  const size_t lds = conv_lds_bytes(b.max_kernel, jac);
   const int tc = (int)std::min<int64_t>(b.ntile, 2 * kPersist);
-  if (jac)
-    hipLaunchKernelGGL(conv_kernel<true>, dim3(tc), dim3(256), lds, stream, b, s, out);
-  else
-    hipLaunchKernelGGL(conv_kernel<false>, dim3(tc), dim3(256), lds, stream, b, s, out);
+  switch (mode) {
+    case EPI_OUT:
+      hipLaunchKernelGGL(conv_kernel<EPI_OUT>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      return;
+    case EPI_NORMAL:
+      hipLaunchKernelGGL(conv_kernel<EPI_NORMAL>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      break;
+    case EPI_RPP:
+      hipLaunchKernelGGL(conv_kernel<EPI_RPP>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      break;
+    default:
+      hipLaunchKernelGGL(conv_kernel<EPI_CHI2>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      break;
+  }
+  hipLaunchKernelGGL(tile_reduce_kernel, dim3(std::min(b.ngal, 2 * kPersist)), dim3(256), 0, stream,
+                     b, s, mode);
 }
 
 // one render stage over the fits active in `stage`; parts: 1 planes, 2 conv
-int run_stage(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
+int run_stage(const apx_batch& b, const Scratch& s, int stage, int mode, const double* rep_override,
               float* out, hipStream_t stream, int parts = 3) {
   if (parts & 1) render_part(b, s, stage, rep_override, stream);
-  if (parts & 2) conv_part(b, s, stage, out, stream);
+  if (parts & 2) conv_part(b, s, mode, out, stream);
   return check_launch("batch stage");
 }
 
@@ -871,6 +1157,8 @@ int batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* b) {
     tile += (int64_t)((G.nx + 63) / 64) * ((G.ny + kRYF - 1) / kRYF);
     kmax = std::max(kmax, std::max(G.ph, G.pw) + 2);
   }
+  if (conv_lds_bytes(kmax, true) + kConvStaticLds > 65536)
+    return set_error(APX_E_LIMIT, "batch_plan: psf too large for the batched Jacobian convolution");
   if (work > (int64_t)INT32_MAX || chunk > (int64_t)INT32_MAX || tile > (int64_t)INT32_MAX)
     return set_error(APX_E_LIMIT, "batch_plan: batch too large");
   b->ngal = ngal;
@@ -905,7 +1193,7 @@ int batch_render(const apx_batch* b, const double* rep, float* out, int32_t part
   if (!rep || !out) return set_error(APX_E_INVALID, "batch_render: null pointer");
   if (parts < 1 || parts > 3) return set_error(APX_E_INVALID, "batch_render: parts must be 1, 2 or 3");
   const Scratch s = carve(*b);
-  return run_stage(*b, s, ST_ALL, rep, out, stream, parts);
+  return run_stage(*b, s, ST_ALL, EPI_OUT, rep, out, stream, parts);
 }
 
 int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream) {
@@ -921,19 +1209,15 @@ int batch_tick(const apx_batch* b, hipStream_t stream) {
   if (rc) return rc;
   const Scratch s = carve(*b);
   const int gb = (b->ngal + 255) / 256;
-  const int fb = std::min(b->ngal, 2 * kPersist);
-  // Jacobian stage for fits starting a step
+  // Jacobian stage for fits starting a step: model0, J, J^T W J, J^T W r, chi2(m0)
   hipLaunchKernelGGL(stage_eval_kernel, dim3(gb), dim3(256), 0, stream, *b);
-  if ((rc = run_stage(*b, s, ST_JAC, nullptr, b->model0, stream))) return rc;
-  hipLaunchKernelGGL(normal_kernel, dim3(fb), dim3(256), 0, stream, *b, s);
+  if ((rc = run_stage(*b, s, ST_JAC, EPI_NORMAL, nullptr, b->model0, stream))) return rc;
   hipLaunchKernelGGL(lm_begin_kernel, dim3(gb), dim3(256), 0, stream, *b);
-  // one sub-step
+  // one sub-step: probe point (rpp), then the trial point (chi2)
   hipLaunchKernelGGL(lm_solve_h_kernel, dim3(gb), dim3(256), 0, stream, *b);
-  if ((rc = run_stage(*b, s, ST_PROBE, nullptr, b->model1, stream))) return rc;
-  hipLaunchKernelGGL(rpp_kernel, dim3(fb), dim3(256), 0, stream, *b, s);
+  if ((rc = run_stage(*b, s, ST_PROBE, EPI_RPP, nullptr, b->model1, stream))) return rc;
   hipLaunchKernelGGL(lm_solve_a_kernel, dim3(gb), dim3(256), 0, stream, *b);
-  if ((rc = run_stage(*b, s, ST_TRIAL, nullptr, b->model1, stream))) return rc;
-  hipLaunchKernelGGL(chi2_kernel, dim3(fb), dim3(256), 0, stream, *b, s);
+  if ((rc = run_stage(*b, s, ST_TRIAL, EPI_CHI2, nullptr, b->model1, stream))) return rc;
   hipLaunchKernelGGL(lm_decide_kernel, dim3(gb), dim3(256), 0, stream, *b);
   return check_launch("batch_tick");
 }

@@ -62,13 +62,17 @@ struct GalEval {
 };
 
 struct Scratch {
-  int32_t* cnt;       // [0] fits, [1] chunks, [2] tiles, [3] refined pixels
+  int32_t* cnt;       // [0] fits, [1] chunks, [2] tiles, [3] level-2 pixels,
+                      // [4] selection redos, [5] level-2 redos
   int32_t* gal_list;  // ngal
   int2* chunk_list;   // nchunk
   int2* tile_list;    // ntile
   int32_t* gal_cnt;   // ngal: pixels each fit sends to the level-2 quadrature
   int32_t* gal_off;   // ngal + 1: exclusive prefix of gal_cnt
   int32_t* deep;      // nwork: per-fit segments (at work_off) of those pixels
+  int32_t* gal_rcnt;  // ngal: pixels each fit sends to a float64 redo pass
+  int32_t* gal_roff;  // ngal + 1: exclusive prefix of gal_rcnt
+  int32_t* redo;      // nwork: per-fit segments of those pixels
   float* mid;         // nwork: midpoint samples
   double* part;       // ntile * kPart: per-tile partial sums of the reductions
   GalEval* ev;        // ngal
@@ -94,6 +98,9 @@ __host__ inline int64_t scratch_layout(const apx_batch& b, Scratch* s) {
   t.gal_cnt = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
   t.gal_off = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
   t.deep = reinterpret_cast<int32_t*>(take(4 * b.nwork));
+  t.gal_rcnt = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
+  t.gal_roff = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
+  t.redo = reinterpret_cast<int32_t*>(take(4 * b.nwork));
   t.mid = reinterpret_cast<float*>(take(4 * b.nwork));
   t.part = reinterpret_cast<double*>(take(8 * kPart * b.ntile));
   t.ev = reinterpret_cast<GalEval*>(take((int64_t)sizeof(GalEval) * b.ngal));
@@ -133,6 +140,7 @@ __global__ __launch_bounds__(256) void build_kernel(apx_batch b, Scratch s, int 
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= b.ngal) return;
   s.gal_cnt[g] = 0;
+  s.gal_rcnt[g] = 0;
   const int ph = st_of(b, g)->phase;
   const bool active = stage == ST_ALL || (stage == ST_JAC ? (ph == PH_INIT || ph == PH_NEEDJ)
                                                          : ph == PH_SUB);
@@ -339,9 +347,97 @@ __global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s) {
   }
 }
 
-// curvature selection (_model_methods.py:87-97, 155-166) and the level-1
-// quadrature of each selected pixel, one lane per pixel; pixels the level-1
-// test sends deeper go to their fit's segment of the `deep` list
+// Outcome of one pixel's selection and level-1 test.  Instantiated with F64
+// false a test too close to call in float32 gives PX_UNDECIDED and writes
+// nothing; the pixel then goes to its fit's redo list and is taken again by
+// the F64 instantiation, whose float64 path keeps the register use of the
+// common float32 kernels low.
+enum : int { PX_DONE = 0, PX_DEEPER = 1, PX_UNDECIDED = 2 };
+
+// curvature selection (_model_methods.py:87-97, 155-166) of working pixel
+// `pix` and, if selected, its level-1 quadrature (operations.py:210-228)
+template <bool GRAD, bool F64>
+__device__ __forceinline__ int select_pixel(const apx_batch& b, const Scratch& s,
+                                            const apx_batch_galaxy& G, const GalEval& E,
+                                            const QuadTab& t, int pix) {
+  const int i = pix % E.nxw, j = pix / E.nxw;
+  const int ii = min(max(i, 1), E.nxw - 2), jj = min(max(j, 1), E.nyw - 2);
+  const float* v = s.mid + G.work_off + (int64_t)jj * E.nxw + ii;
+  const float l = v[-1], r = v[1], u = v[-E.nxw], d = v[E.nxw], m = v[0];
+  const float metric = fabsf(l + r + u + d - 4.0f * m);
+  const float scale = fabsf(l) + fabsf(r) + fabsf(u) + fabsf(d) + 4.0f * fabsf(m);
+  const double band = 1.6e-5 * (double)scale + 2.0e-7 * E.thr;
+  bool sel = false;
+  if ((double)metric > E.thr + band) {
+    sel = true;
+  } else if ((double)metric >= E.thr - band) {  // float64 decision
+    if (!F64) return PX_UNDECIDED;
+    double X, Y;
+    auto at = [&](int a, int c) {
+      pixel_xy(E.pd, E.ci, E.cj, a, c, X, Y);
+      return eval_profile<double, false>(E.pd, X, Y, 1.0, nullptr);
+    };
+    const double mc = at(ii, jj);
+    const double curv = at(ii - 1, jj) + at(ii + 1, jj) + at(ii, jj - 1) + at(ii, jj + 1) - 4.0 * mc;
+    const double mid = (i == ii && j == jj) ? mc : at(i, j);
+    sel = fabs(mid - (mid + curv)) > E.thr;
+  }
+  float* pl = b.planes + G.work_off + pix;
+  if (!sel) {
+    pl[0] = s.mid[G.work_off + pix];
+    return PX_DONE;
+  }
+  float X0, Y0, gs[APX_NSLOT], centre = 0.0f;
+  pixel_xy(E.pf, E.ci, E.cj, i, j, X0, Y0);
+  const float r1 = quad_f<GRAD>(E.pf, t, X0, Y0, 0.5f, 1.0f, gs, centre);
+  if (t.max_depth > 1) {
+    const int dec = band_test(r1, centre, E.thr);
+    bool deeper = dec == 1;
+    if (dec < 0) {
+      if (!F64) return PX_UNDECIDED;
+      double X, Y, cd = 0.0;
+      pixel_xy(E.pd, E.ci, E.cj, i, j, X, Y);
+      const double rd = quad_d(E.pd, t, X, Y, 0.5, 1.0, cd);
+      deeper = fabs(rd - cd) > E.thr;
+    }
+    if (deeper) return PX_DEEPER;
+  }
+  pl[0] = r1;
+  if (GRAD) {
+#pragma unroll
+    for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = gs[c + 1] * (float)E.dvdr[c + 1];
+  }
+  return PX_DONE;
+}
+
+// appends `value` for every lane with `flag` to a segment whose length is
+// `*counter`: one atomic per wave (all lanes share the counter)
+__device__ __forceinline__ void wave_append(bool flag, int32_t* counter, int32_t* seg, int value,
+                                            int lane) {
+  const uint64_t ballot = __ballot(flag);
+  if (!ballot) return;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(counter, (int)__popcll(ballot));
+  base = __shfl(base, 0, 64);
+  if (flag) {
+    const uint64_t below = ballot & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    seg[base + (int)__popcll(below)] = value;
+  }
+}
+
+// fit owning entry k of a list whose per-fit segments start at off[g]
+// (exclusive prefix, off[ngal] = total > k)
+__device__ __forceinline__ int locate(const int32_t* off, int ngal, int k) {
+  int lo = 0, hi = ngal;  // off[lo] <= k < off[hi]
+  while (hi - lo > 1) {
+    const int md = (lo + hi) >> 1;
+    if (off[md] <= k) lo = md;
+    else hi = md;
+  }
+  return lo;
+}
+
+// float32 selection of every listed chunk, one lane per pixel
 template <bool GRAD>
 __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, QuadTab t) {
   const int lane = threadIdx.x & 63;
@@ -350,77 +446,36 @@ __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, Qua
     const apx_batch_galaxy G = b.gal[e.x];
     const GalEval E = s.ev[e.x];
     const int pix = e.y * kChunk + threadIdx.x;
-    bool deeper = false;
-    if (pix < E.nxw * E.nyw) {
-      const int i = pix % E.nxw, j = pix / E.nxw;
-      const int ii = min(max(i, 1), E.nxw - 2), jj = min(max(j, 1), E.nyw - 2);
-      const float* v = s.mid + G.work_off + (int64_t)jj * E.nxw + ii;
-      const float l = v[-1], r = v[1], u = v[-E.nxw], d = v[E.nxw], m = v[0];
-      const float metric = fabsf(l + r + u + d - 4.0f * m);
-      const float scale = fabsf(l) + fabsf(r) + fabsf(u) + fabsf(d) + 4.0f * fabsf(m);
-      const double band = 1.6e-5 * (double)scale + 2.0e-7 * E.thr;
-      bool sel = false;
-      if ((double)metric > E.thr + band) {
-        sel = true;
-      } else if ((double)metric >= E.thr - band) {  // float64 re-decision
-        double X, Y;
-        auto at = [&](int a, int c) {
-          pixel_xy(E.pd, E.ci, E.cj, a, c, X, Y);
-          return eval_profile<double, false>(E.pd, X, Y, 1.0, nullptr);
-        };
-        const double mc = at(ii, jj);
-        const double curv = at(ii - 1, jj) + at(ii + 1, jj) + at(ii, jj - 1) + at(ii, jj + 1) - 4.0 * mc;
-        const double mid = (i == ii && j == jj) ? mc : at(i, j);
-        sel = fabs(mid - (mid + curv)) > E.thr;
-      }
-      float* pl = b.planes + G.work_off + pix;
-      if (!sel) {
-        pl[0] = s.mid[G.work_off + pix];
-      } else {
-        float X0, Y0, gs[APX_NSLOT], centre = 0.0f;
-        pixel_xy(E.pf, E.ci, E.cj, i, j, X0, Y0);
-        const float r1 = quad_f<GRAD>(E.pf, t, X0, Y0, 0.5f, 1.0f, gs, centre);
-        if (t.max_depth > 1) {
-          const int dec = band_test(r1, centre, E.thr);
-          if (dec < 0) {
-            double X, Y, cd = 0.0;
-            pixel_xy(E.pd, E.ci, E.cj, i, j, X, Y);
-            const double rd = quad_d(E.pd, t, X, Y, 0.5, 1.0, cd);
-            deeper = fabs(rd - cd) > E.thr;
-          } else {
-            deeper = dec == 1;
-          }
-        }
-        if (!deeper) {
-          pl[0] = r1;
-          if (GRAD) {
-#pragma unroll
-            for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = gs[c + 1] * (float)E.dvdr[c + 1];
-          }
-        }
-      }
-    }
-    const uint64_t ballot = __ballot(deeper);
-    if (ballot) {  // one atomic per wave on the fit's own counter
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&s.gal_cnt[e.x], (int)__popcll(ballot));
-      base = __shfl(base, 0, 64);
-      if (deeper) {
-        const uint64_t below = ballot & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-        s.deep[G.work_off + base + (int)__popcll(below)] = pix;
-      }
-    }
+    const int res = pix < E.nxw * E.nyw ? select_pixel<GRAD, false>(b, s, G, E, t, pix) : PX_DONE;
+    wave_append(res == PX_DEEPER, &s.gal_cnt[e.x], s.deep + G.work_off, pix, lane);
+    wave_append(res == PX_UNDECIDED, &s.gal_rcnt[e.x], s.redo + G.work_off, pix, lane);
   }
 }
 
-// exclusive prefix of the per-fit counts: one workgroup of 1024
-__global__ __launch_bounds__(1024) void scan_kernel(apx_batch b, Scratch s) {
+// the pixels select_kernel left undecided, with float64 decisions
+template <bool GRAD>
+__global__ __launch_bounds__(256) void select_redo_kernel(apx_batch b, Scratch s, QuadTab t) {
+  const int total = s.cnt[4];
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
+    const int g = locate(s.gal_roff, b.ngal, k);
+    const apx_batch_galaxy G = b.gal[g];
+    const int pix = s.redo[G.work_off + (k - s.gal_roff[g])];
+    if (select_pixel<GRAD, true>(b, s, G, s.ev[g], t, pix) == PX_DEEPER)
+      s.deep[G.work_off + atomicAdd(&s.gal_cnt[g], 1)] = pix;
+  }
+}
+
+// exclusive prefix of n per-fit counts (one workgroup of 1024): off[0..n],
+// *total = off[n]; the counts are cleared after reading when `clear`
+__global__ __launch_bounds__(1024) void scan_kernel(int n, int32_t* cnt, int32_t* off,
+                                                    int32_t* total, int clear) {
   __shared__ int32_t wsums[16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int carry = 0;
-  for (int base = 0; base < b.ngal; base += 1024) {
+  for (int base = 0; base < n; base += 1024) {
     const int g = base + threadIdx.x;
-    const int v = g < b.ngal ? s.gal_cnt[g] : 0;
+    const int v = g < n ? cnt[g] : 0;
+    if (clear && g < n) cnt[g] = 0;
     int x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -429,48 +484,45 @@ __global__ __launch_bounds__(1024) void scan_kernel(apx_batch b, Scratch s) {
     }
     if (lane == 63) wsums[wv] = x;
     __syncthreads();
-    int pre = 0, total = 0;
+    int pre = 0, tot = 0;
     for (int w = 0; w < 16; ++w) {
       pre += w < wv ? wsums[w] : 0;
-      total += wsums[w];
+      tot += wsums[w];
     }
-    if (g < b.ngal) s.gal_off[g] = carry + pre + x - v;
-    carry += total;
+    if (g < n) off[g] = carry + pre + x - v;
+    carry += tot;
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    s.gal_off[b.ngal] = carry;
-    s.cnt[3] = carry;
+    off[n] = carry;
+    *total = carry;
   }
 }
 
-// levels 2 and 3 of the quadrature: a group of GS lanes per listed pixel, one
-// lane per level-2 sub-pixel; each sub-pixel the level-2 test flags is then
-// integrated by the whole group (operations.py:229-247)
-template <bool GRAD, int GS>
+// levels 2 and 3 of the quadrature (operations.py:229-247): a group of GS
+// lanes per listed pixel, one lane per level-2 sub-pixel; each sub-pixel the
+// level-2 test flags is then integrated by the whole group.  F64 false: the
+// deep list, and a pixel with an undecided sub-pixel test goes to the redo
+// list unwritten; F64 true: the redo list, float64 decisions.
+template <bool GRAD, int GS, bool F64>
 __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, QuadTab t) {
   constexpr int NG = 64 / GS;
   const int lane = threadIdx.x & 63, gl = lane % GS, grp = lane / GS;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
-  const int total = s.cnt[3];
+  const int total = F64 ? s.cnt[5] : s.cnt[3];
+  const int32_t* off = F64 ? s.gal_roff : s.gal_off;
+  const int32_t* list = F64 ? s.redo : s.deep;
   const int g_ = t.g, gg = t.gg, nq = gg * t.nn;
   const float ig = 1.0f / (float)g_;
   const float sub0 = -(float)(g_ - 1) / (2.0f * (float)g_), subd = ig;
-  const double igd = 1.0 / (double)g_;
-  const double sub0d = -(double)(g_ - 1) / (2.0 * g_);
+  const uint64_t gmask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << (grp * GS));
   for (int k0 = wave * NG; k0 < total; k0 += nwaves * NG) {
     const int k = k0 + grp;
     const bool active = k < total;
     int gal = 0, pix = 0;
     if (active) {
-      int lo = 0, hi = b.ngal;  // gal_off[lo] <= k < gal_off[hi]
-      while (hi - lo > 1) {
-        const int md = (lo + hi) >> 1;
-        if (s.gal_off[md] <= k) lo = md;
-        else hi = md;
-      }
-      gal = lo;
-      pix = s.deep[b.gal[gal].work_off + (k - s.gal_off[gal])];
+      gal = locate(off, b.ngal, k);
+      pix = list[b.gal[gal].work_off + (k - off[gal])];
     }
     const GalEval* E = s.ev + gal;  // fit 0's constants for an idle group: read, never used
     const Prof<float> pf = E->pf;
@@ -482,40 +534,40 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
 #pragma unroll
     for (int q = 0; q <= APX_NSLOT; ++q) acc[q] = 0.0;
     // level 2
-    bool deeper = false;
-    float x2 = 0.0f, y2 = 0.0f;
+    int dec = 0;
+    float x2 = 0.0f, y2 = 0.0f, r2 = 0.0f, gs[APX_NSLOT];
     if (active && gl < gg) {
       const float a = sub0 + subd * (float)(gl % g_), c = sub0 + subd * (float)(gl / g_);
       x2 = X0 + pf.P00 * a + pf.P01 * c;
       y2 = Y0 + pf.P10 * a + pf.P11 * c;
-      float gs[APX_NSLOT], c2 = 0.0f;
-      const float r2 = quad_f<GRAD>(pf, t, x2, y2, 0.5f * ig, ig * ig, gs, c2);
+      float c2 = 0.0f;
+      r2 = quad_f<GRAD>(pf, t, x2, y2, 0.5f * ig, ig * ig, gs, c2);
       if (t.max_depth > 2) {
         const double thr2 = E->thr * (double)gg;
-        const int dec = band_test(r2, c2, thr2);
-        if (dec < 0) {
-          double X, Y, cd = 0.0;
-          pixel_xy(E->pd, ci, cj, pi, pj, X, Y);
+        dec = band_test(r2, c2, thr2);
+        if (F64 && dec < 0) {
+          const double igd = 1.0 / (double)g_, sub0d = -(double)(g_ - 1) / (2.0 * g_);
           const double ad = sub0d + igd * (gl % g_), bd = sub0d + igd * (gl / g_);
           const Prof<double>& pd = E->pd;
+          double X, Y, cd = 0.0;
+          pixel_xy(pd, ci, cj, pi, pj, X, Y);
           const double rd = quad_d(pd, t, X + pd.P00 * ad + pd.P01 * bd, Y + pd.P10 * ad + pd.P11 * bd,
                                    0.5 * igd, igd * igd, cd);
-          deeper = fabs(rd - cd) > thr2;
-        } else {
-          deeper = dec == 1;
-        }
-      }
-      if (!deeper) {
-        acc[0] = (double)r2;
-        if (GRAD) {
-#pragma unroll
-          for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] = (double)gs[q];
+          dec = fabs(rd - cd) > thr2 ? 1 : 0;
         }
       }
     }
+    const bool undecided = !F64 && (__ballot(dec < 0) & gmask) != 0ull;
+    const bool deeper = dec == 1 && !undecided;
+    if (active && gl < gg && dec == 0 && !undecided) {
+      acc[0] = (double)r2;
+      if (GRAD) {
+#pragma unroll
+        for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] = (double)gs[q];
+      }
+    }
     // level 3: the group integrates each flagged sub-pixel in turn
-    const uint64_t bal = __ballot(deeper);
-    uint64_t todo = GS == 64 ? bal : ((bal >> (grp * GS)) & ((1ull << GS) - 1ull));
+    uint64_t todo = (__ballot(deeper) & gmask) >> (GS == 64 ? 0 : grp * GS);
     const float sc3 = ig * ig * ig * ig, h3 = 0.5f * ig * ig;
     while (__ballot(todo != 0ull)) {
       int sp = -1;
@@ -549,11 +601,16 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
       for (int o = GS / 2; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
     }
     if (active && gl == 0) {
-      float* pl = b.planes + b.gal[gal].work_off + pix;
-      pl[0] = (float)acc[0];
-      if (GRAD) {
+      const int64_t wo = b.gal[gal].work_off;
+      if (undecided) {
+        s.redo[wo + atomicAdd(&s.gal_rcnt[gal], 1)] = pix;
+      } else {
+        float* pl = b.planes + wo + pix;
+        pl[0] = (float)acc[0];
+        if (GRAD) {
 #pragma unroll
-        for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = (float)(acc[1 + c + 1] * E->dvdr[c + 1]);
+          for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = (float)(acc[1 + c + 1] * E->dvdr[c + 1]);
+        }
       }
     }
   }
@@ -564,39 +621,35 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
 // folded into the epilogue as per-tile partial sums:
 //   EPI_OUT     value -> out
 //   EPI_NORMAL  value -> out (model0), centre derivatives -> J cols 0, 1,
-//               partial planes -> J cols 2..6; J^T W J, J^T W (Y - m0),
-//               chi2(m0) of the tile                        (lm.py:256-260)
+//               partial planes -> J cols 2..6; normal_kernel then forms
+//               J^T W J, J^T W (Y - m0), chi2(m0) per tile  (lm.py:256-260)
 //   EPI_RPP     J^T (2 / d) (W (m(x + d h) - m0) / d - W J h), d = 0.1
 //                                                           (lm.py:276-280)
 //   EPI_CHI2    sum W (Y - m)^2 of the tile                 (lm.py:290-292)
 enum : int { EPI_OUT = 0, EPI_NORMAL = 1, EPI_RPP = 2, EPI_CHI2 = 3 };
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-__host__ __device__ inline int conv_stage_floats(int kmax, bool jac) {
+// dynamic LDS of conv_kernel: the staged input tile, reused for the
+// cross-wave reduction of one kernel's outputs at a time
+__host__ __device__ inline int conv_stage_floats(int kmax) {
   const int tile = 8 * fast_nt(kmax) * (64 + kmax - 1);
-  const int red = 4 * (jac ? 3 : 1) * kRYF * 64;
+  const int red = 4 * kRYF * 64;
   return tile > red ? tile : red;
 }
+__host__ inline size_t conv_lds_bytes(int kmax) { return sizeof(float) * (size_t)conv_stage_floats(kmax); }
+constexpr size_t kConvStaticLds = sizeof(double) * 4 * 8;
 
-// dynamic LDS of conv_kernel: staging / reduction area, plus the J and model
-// tile of the normal-equation epilogue
-__host__ inline size_t conv_lds_bytes(int kmax, bool jac) {
-  return sizeof(float) * ((size_t)conv_stage_floats(kmax, jac) + (jac ? (NP + 1) * kRYF * 64 : 0));
-}
-constexpr size_t kConvStaticLds = sizeof(double) * 4 * (4 * 64 + 24);
-
+// After the cross-wave reduction each wave owns output rows w and w + 4 of
+// the tile (lane = column) for every kernel and channel, so the probe and
+// trial epilogues work from the two values each lane holds.
 template <int MODE>
 __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float* out) {
   extern __shared__ float smem[];
-  __shared__ double redd[4][4 * 64 + 24];  // kConvStaticLds
+  __shared__ double redd[4][8];  // kConvStaticLds
   constexpr bool JAC = MODE == EPI_NORMAL;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wu = __builtin_amdgcn_readfirstlane(w);
   constexpr int NKmax = JAC ? 3 : 1;
   constexpr int nchan = JAC ? kNChan : 1;
-  float* tJ = smem + conv_stage_floats(b.max_kernel, JAC);  // [NP][kRYF][64]
-  float* tM = tJ + NP * kRYF * 64;                          // [kRYF][64]
   for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
     const int2 e = s.tile_list[k];
     const int g = __builtin_amdgcn_readfirstlane(e.x);
@@ -610,15 +663,15 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
     // output (ox, oy) sits at working (bx + ox, by + oy); taps span +-(k-1)/2 = border
     const float* kt = b.taps + G.tap_off;
     const int ox = ox0 + lane;
-    double racc[NP];
+    bool valid[2];
+    int64_t pp[2];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) racc[i] = 0.0;
-    double hv[NP];
-    if (MODE == EPI_RPP) {
-      const GalState* st = st_of(b, g);
-#pragma unroll
-      for (int i = 0; i < NP; ++i) hv[i] = st->h[i];
+    for (int sl = 0; sl < 2; ++sl) {
+      const int oy = oy0 + w + 4 * sl;
+      valid[sl] = ox < G.nx && oy < G.ny;
+      pp[sl] = G.pix_off + (int64_t)oy * G.nx + ox;
     }
+    float mv[2] = {0.0f, 0.0f};
     for (int c = 0; c < nchan; ++c) {
       const float* plane = b.planes + (int64_t)c * b.nwork + G.work_off;
       __syncthreads();
@@ -634,107 +687,140 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
 #pragma unroll
         for (int r = 0; r < kRYF; ++r) acc[kk][r] = 0.0f;
       conv_accumulate<NKmax>(smem, LW, lane, wu, nT, kw, kts, nk, kt, acc);
-      __syncthreads();
-      float* red = smem;
 #pragma unroll
-      for (int kk = 0; kk < NKmax; ++kk)
-        if (kk < nk)
+      for (int kk = 0; kk < NKmax; ++kk) {
+        if (kk >= nk) break;
+        __syncthreads();
 #pragma unroll
-          for (int r = 0; r < kRYF; ++r) red[((w * NKmax + kk) * kRYF + r) * 64 + lane] = acc[kk][r];
-      __syncthreads();
-      for (int q = w; q < nk * kRYF; q += 4) {
-        const int kk = q / kRYF, r = q % kRYF;
-        float sum = 0.0f;
-#pragma unroll
-        for (int ww = 0; ww < 4; ++ww) sum += red[((ww * NKmax + kk) * kRYF + r) * 64 + lane];
-        const int oy = oy0 + r;
-        const bool valid = ox < G.nx && oy < G.ny;
+        for (int r = 0; r < kRYF; ++r) smem[(w * kRYF + r) * 64 + lane] = acc[kk][r];
+        __syncthreads();
         const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
-        if (JAC) {
-          if (col < 0) tM[r * 64 + lane] = valid ? sum : 0.0f;
-          else tJ[(col * kRYF + r) * 64 + lane] = valid ? sum : 0.0f;
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+          const int r = w + 4 * sl;
+          float sum = 0.0f;
+#pragma unroll
+          for (int ww = 0; ww < 4; ++ww) sum += smem[(ww * kRYF + r) * 64 + lane];
+          if (JAC) {
+            if (valid[sl]) {
+              if (col < 0) out[pp[sl]] = sum;
+              else b.jac[pp[sl] * NP + col] = sum;
+            }
+          } else {
+            mv[sl] = sum;
+            if (MODE == EPI_OUT && valid[sl]) out[pp[sl]] = sum;
+          }
         }
-        if (!valid) continue;
-        const int64_t p = G.pix_off + (int64_t)oy * G.nx + ox;
-        if (MODE == EPI_OUT || JAC) {
-          if (col < 0) out[p] = sum;
-          else if (JAC) b.jac[p * NP + col] = sum;
-        } else if (MODE == EPI_RPP) {
-          constexpr double d = 0.1;
+      }
+    }
+    // ---- epilogue: the tile's partial sums, reduced in float64
+    double* part = s.part + tile * kPart;
+    if (MODE == EPI_OUT || MODE == EPI_NORMAL) continue;  // normal equations: normal_kernel
+    // per lane two pixels, summed in float64
+    constexpr int W = MODE == EPI_RPP ? NP : 1;
+    {
+      double rd[W];
+#pragma unroll
+      for (int i = 0; i < W; ++i) rd[i] = 0.0;
+      if (MODE == EPI_RPP) {
+        const GalState* st = st_of(b, g);
+        constexpr double d = 0.1;
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+          if (!valid[sl]) continue;
           float jr[NP];
           double jh = 0.0;
 #pragma unroll
           for (int i = 0; i < NP; ++i) {
-            jr[i] = b.jac[p * NP + i];
-            jh += (double)jr[i] * hv[i];
+            jr[i] = b.jac[pp[sl] * NP + i];
+            jh += (double)jr[i] * st->h[i];
           }
-          const double wt = (double)b.weight[p];
-          const double v = (2.0 / d) * (wt * ((double)sum - (double)b.model0[p]) / d - wt * jh);
+          const double wt = (double)b.weight[pp[sl]];
+          const double v = (2.0 / d) * (wt * ((double)mv[sl] - (double)b.model0[pp[sl]]) / d - wt * jh);
 #pragma unroll
-          for (int i = 0; i < NP; ++i) racc[i] += (double)jr[i] * v;
-        } else {  // EPI_CHI2
-          const double dd = (double)b.data[p] - (double)sum;
-          racc[0] += (double)b.weight[p] * dd * dd;
+          for (int i = 0; i < NP; ++i) rd[i] += (double)jr[i] * v;
+        }
+      } else {  // EPI_CHI2
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+          if (!valid[sl]) continue;
+          const double dd = (double)b.data[pp[sl]] - (double)mv[sl];
+          rd[0] += (double)b.weight[pp[sl]] * dd * dd;
         }
       }
-    }
-    // ---- epilogue: the tile's partial sums
-    double* part = s.part + tile * kPart;
-    if (MODE == EPI_NORMAL) {
-      __syncthreads();  // tJ, tM complete
-      const int r16 = lane & 15, kq = lane >> 4;
-      v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-      float gpart = 0.0f;
-      double cacc = 0.0;
-      for (int stp = 0; stp < 32; ++stp) {
-        const int pp = w * 128 + stp * 4 + kq;  // tile pixel
-        const int r = pp >> 6, lx = pp & 63;
-        const int px = ox0 + lx, py = oy0 + r;
-        float a = 0.0f, wt = 0.0f, res = 0.0f;
-        if (px < G.nx && py < G.ny) {
-          const int64_t p = G.pix_off + (int64_t)py * G.nx + px;
-          const float m = tM[r * 64 + lx];
-          wt = b.weight[p];
-          res = b.data[p] - m;
-          if (r16 < NP) a = tJ[(r16 * kRYF + r) * 64 + lx];
-          if (r16 == 0) {
-            const double dd = (double)b.data[p] - (double)m;
-            cacc += (double)wt * dd * dd;
-          }
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wt * a, acc, 0, 0, 0);
-        gpart = fmaf(a * wt, res, gpart);
-      }
-      // D[4 kq + i][r16] = acc[i]
-#pragma unroll
-      for (int i = 0; i < 4; ++i) redd[w][(4 * kq + i) * 16 + r16] = (double)acc[i];
-      double gd = (double)gpart;
-      gd += __shfl_xor(gd, 16, 64);
-      gd += __shfl_xor(gd, 32, 64);
-      if (lane < 16) redd[w][256 + lane] = gd;
-      cacc = wsum(cacc);
-      if (lane == 0) redd[w][256 + 16] = cacc;
-      __syncthreads();
-      if (threadIdx.x < NP * NP + NP + 1) {
-        const int t = threadIdx.x;
-        const int idx = t < NP * NP ? (t / NP) * 16 + t % NP : (t < NP * NP + NP ? 256 + t - NP * NP : 256 + 16);
-        part[t] = redd[0][idx] + redd[1][idx] + redd[2][idx] + redd[3][idx];
-      }
-      __syncthreads();
-    } else if (MODE == EPI_RPP || MODE == EPI_CHI2) {
-      constexpr int W = MODE == EPI_RPP ? NP : 1;
 #pragma unroll
       for (int i = 0; i < W; ++i) {
-        const double v = wsum(racc[i]);
+        const double v = wsum(rd[i]);
         if (lane == 0) redd[w][i] = v;
       }
-      __syncthreads();
-      if (threadIdx.x < W) {
-        const int i = threadIdx.x;
-        part[i] = redd[0][i] + redd[1][i] + redd[2][i] + redd[3][i];
-      }
-      __syncthreads();
     }
+    __syncthreads();
+    if (threadIdx.x < W) {
+      const int i = threadIdx.x;
+      part[i] = redd[0][i] + redd[1][i] + redd[2][i] + redd[3][i];
+    }
+    __syncthreads();
+  }
+}
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// per tile: J^T W J (MFMA 16x16x4 f32, 4 pixels per step, 128 pixels per
+// wave), J^T W (Y - m0) and chi2(m0), into the tile's partial sums
+__global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
+  __shared__ double redH[4][256];
+  __shared__ double redg[4][16];
+  __shared__ double redc[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r16 = lane & 15, kq = lane >> 4;
+  for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
+    const int2 e = s.tile_list[k];
+    const apx_batch_galaxy G = b.gal[e.x];
+    const int ntx = (G.nx + 63) / 64;
+    const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kRYF;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    float gpart = 0.0f;
+    double cacc = 0.0;
+    for (int stp = 0; stp < 32; ++stp) {
+      const int pt = wv * 128 + stp * 4 + kq;  // tile pixel
+      const int px = ox0 + (pt & 63), py = oy0 + (pt >> 6);
+      float a = 0.0f, wt = 0.0f, res = 0.0f;
+      if (px < G.nx && py < G.ny) {
+        const int64_t p = G.pix_off + (int64_t)py * G.nx + px;
+        const float m = b.model0[p], y = b.data[p];
+        wt = b.weight[p];
+        res = y - m;
+        if (r16 < NP) a = b.jac[p * NP + r16];
+        if (r16 == 0) {
+          const double dd = (double)y - (double)m;
+          cacc += (double)wt * dd * dd;
+        }
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wt * a, acc, 0, 0, 0);
+      gpart = fmaf(a * wt, res, gpart);
+    }
+    // D[4 kq + i][r16] = acc[i]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) redH[wv][(4 * kq + i) * 16 + r16] = (double)acc[i];
+    double gd = (double)gpart;
+    gd += __shfl_xor(gd, 16, 64);
+    gd += __shfl_xor(gd, 32, 64);
+    if (lane < 16) redg[wv][lane] = gd;
+    cacc = wsum(cacc);
+    if (lane == 0) redc[wv] = cacc;
+    __syncthreads();
+    double* part = s.part + (G.tile_off + e.y) * kPart;  // H row-major, g, chi2
+    const int t = threadIdx.x;
+    if (t < NP * NP) {
+      const int idx = (t / NP) * 16 + t % NP;
+      part[t] = redH[0][idx] + redH[1][idx] + redH[2][idx] + redH[3][idx];
+    } else if (t < NP * NP + NP) {
+      const int i = t - NP * NP;
+      part[t] = redg[0][i] + redg[1][i] + redg[2][i] + redg[3][i];
+    } else if (t == NP * NP + NP) {
+      part[t] = redc[0] + redc[1] + redc[2] + redc[3];
+    }
+    __syncthreads();
   }
 }
 
@@ -1067,6 +1153,18 @@ QuadTab quad_table(const apx_batch& b) {
   return t;
 }
 
+template <int GS>
+void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac,
+                      hipStream_t stream) {
+  const dim3 ig(kPersist / 2);
+  if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, false>), ig, dim3(256), 0, stream, b, s, t);
+  else hipLaunchKernelGGL((integrate_kernel<false, GS, false>), ig, dim3(256), 0, stream, b, s, t);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
+                     s.cnt + 5, 0);
+  if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true>), ig, dim3(256), 0, stream, b, s, t);
+  else hipLaunchKernelGGL((integrate_kernel<false, GS, true>), ig, dim3(256), 0, stream, b, s, t);
+}
+
 // work lists + render planes of the fits active in `stage`
 void render_part(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
                  hipStream_t stream) {
@@ -1085,22 +1183,23 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
     hipLaunchKernelGGL(init_kernel<false>, dim3(pc), dim3(kChunk), 0, stream, b, s);
     hipLaunchKernelGGL(select_kernel<false>, dim3(pc), dim3(256), 0, stream, b, s, t);
   }
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b, s);
+  // float64 redo of undecided selections, then the level-2/3 quadrature of
+  // the deep list, then its undecided pixels in float64
   const dim3 ig(kPersist / 2);
-  if (t.gg <= 32) {
-    if (jac) hipLaunchKernelGGL((integrate_kernel<true, 32>), ig, dim3(256), 0, stream, b, s, t);
-    else hipLaunchKernelGGL((integrate_kernel<false, 32>), ig, dim3(256), 0, stream, b, s, t);
-  } else {
-    if (jac) hipLaunchKernelGGL((integrate_kernel<true, 64>), ig, dim3(256), 0, stream, b, s, t);
-    else hipLaunchKernelGGL((integrate_kernel<false, 64>), ig, dim3(256), 0, stream, b, s, t);
-  }
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
+                     s.cnt + 4, 1);
+  if (jac) hipLaunchKernelGGL(select_redo_kernel<true>, ig, dim3(256), 0, stream, b, s, t);
+  else hipLaunchKernelGGL(select_redo_kernel<false>, ig, dim3(256), 0, stream, b, s, t);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_cnt, s.gal_off,
+                     s.cnt + 3, 0);
+  if (t.gg <= 32) launch_integrate<32>(b, s, t, jac, stream);
+  else launch_integrate<64>(b, s, t, jac, stream);
 }
 
 // convolution of the listed tiles (epilogue `mode`), then for the LM stages
 // the per-fit sums of the tile partials
 void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream) {
-  const bool jac = mode == EPI_NORMAL;
-  const size_t lds = conv_lds_bytes(b.max_kernel, jac);
+  const size_t lds = conv_lds_bytes(b.max_kernel);
   const int tc = (int)std::min<int64_t>(b.ntile, 2 * kPersist);
   switch (mode) {
     case EPI_OUT:
@@ -1108,6 +1207,7 @@ void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipSt
       return;
     case EPI_NORMAL:
       hipLaunchKernelGGL(conv_kernel<EPI_NORMAL>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      hipLaunchKernelGGL(normal_kernel, dim3(tc), dim3(256), 0, stream, b, s);
       break;
     case EPI_RPP:
       hipLaunchKernelGGL(conv_kernel<EPI_RPP>, dim3(tc), dim3(256), lds, stream, b, s, out);
@@ -1157,7 +1257,7 @@ int batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* b) {
     tile += (int64_t)((G.nx + 63) / 64) * ((G.ny + kRYF - 1) / kRYF);
     kmax = std::max(kmax, std::max(G.ph, G.pw) + 2);
   }
-  if (conv_lds_bytes(kmax, true) + kConvStaticLds > 65536)
+  if (conv_lds_bytes(kmax) + kConvStaticLds > 65536)
     return set_error(APX_E_LIMIT, "batch_plan: psf too large for the batched Jacobian convolution");
   if (work > (int64_t)INT32_MAX || chunk > (int64_t)INT32_MAX || tile > (int64_t)INT32_MAX)
     return set_error(APX_E_LIMIT, "batch_plan: batch too large");

@@ -62,6 +62,8 @@ SIGNATURES = {
     "apx_batch_scratch_bytes": (ctypes.c_int64, [ctypes.POINTER(Batch)]),
     "apx_batch_render": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                         ctypes.c_void_p]),
+    "apx_batch_jacobian": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_int32, ctypes.c_void_p]),
     "apx_batch_start": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
     "apx_batch_tick": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p]),
     "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 6),
@@ -179,6 +181,20 @@ class BatchFit:
         L.check(self.lib.apx_batch_render(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), parts,
                                           L.stream()))
         return out
+
+    def jacobian(self, rep, out=None, parts=3):
+        """All cutouts rendered at ``rep`` into ``out`` (as render) and their
+        Jacobians with respect to the 7 representation-space parameters,
+        returned as an (npix, 7) view of the batch's J buffer (pixel-major,
+        cutouts concatenated).  ``parts`` as in render.  The LM state of the
+        fits is left untouched."""
+        rep = torch.as_tensor(rep, dtype=torch.float64, device=self.data.device).contiguous()
+        self._render_rep = rep
+        if out is None:
+            out = torch.zeros(self.npix, dtype=torch.float32, device=self.data.device)
+        L.check(self.lib.apx_batch_jacobian(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), parts,
+                                            L.stream()))
+        return out, self.jac.view(self.npix, NP)
 
     def conv_flops(self) -> float:
         """Algorithmic flops of one value convolution of every cutout:

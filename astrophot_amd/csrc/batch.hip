@@ -37,7 +37,7 @@ constexpr int kPersist = 2048;
 constexpr int kNChan = 6;  // value + q, PA, n, Re, Ie partials
 
 enum : int32_t { PH_INIT = 0, PH_NEEDJ = 1, PH_SUB = 2, PH_DONE = 3 };
-enum : int32_t { ST_JAC = 0, ST_PROBE = 1, ST_TRIAL = 2, ST_ALL = 3 };
+enum : int32_t { ST_JAC = 0, ST_PROBE = 1, ST_TRIAL = 2, ST_ALL = 3, ST_ALL_JAC = 4 };
 enum : int32_t { DIR_NONE = 0, DIR_BETTER = 1, DIR_WORSE = 2 };
 enum : int32_t { S_RUN = 0, S_SUCCESS = 1, S_IMMOBILE = 2, S_NOSTEP = 3, S_MAXITER = 4 };
 
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(256) void build_kernel(apx_batch b, Scratch s, int 
   s.gal_cnt[g] = 0;
   s.gal_rcnt[g] = 0;
   const int ph = st_of(b, g)->phase;
-  const bool active = stage == ST_ALL || (stage == ST_JAC ? (ph == PH_INIT || ph == PH_NEEDJ)
-                                                         : ph == PH_SUB);
+  const bool active = stage == ST_ALL || stage == ST_ALL_JAC ||
+                      (stage == ST_JAC ? (ph == PH_INIT || ph == PH_NEEDJ) : ph == PH_SUB);
   if (!active) return;
   const apx_batch_galaxy G = b.gal[g];
   s.gal_list[atomicAdd(&s.cnt[0], 1)] = g;
@@ -177,7 +177,7 @@ __device__ inline void rep_to_values(const double* r, double* v, double* dvdr) {
 __global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, int stage,
                                                       const double* rep_override) {
   __shared__ double red[4];
-  const int nk = stage == ST_JAC ? 3 : 1;
+  const int nk = (stage == ST_JAC || stage == ST_ALL_JAC) ? 3 : 1;
   for (int k = blockIdx.x; k < s.cnt[0]; k += gridDim.x) {
     const int g = s.gal_list[k];
     const apx_batch_galaxy G = b.gal[g];
@@ -1168,7 +1168,7 @@ void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bo
 // work lists + render planes of the fits active in `stage`
 void render_part(const apx_batch& b, const Scratch& s, int stage, const double* rep_override,
                  hipStream_t stream) {
-  const bool jac = stage == ST_JAC;
+  const bool jac = stage == ST_JAC || stage == ST_ALL_JAC;
   const int gb = (b.ngal + 255) / 256;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, s.cnt);
   hipLaunchKernelGGL(build_kernel, dim3(gb), dim3(256), 0, stream, b, s, stage);
@@ -1197,8 +1197,9 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
 }
 
 // convolution of the listed tiles (epilogue `mode`), then for the LM stages
-// the per-fit sums of the tile partials
-void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream) {
+// (`reduce`) the per-fit sums of the tile partials
+void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream,
+               bool reduce) {
   const size_t lds = conv_lds_bytes(b.max_kernel);
   const int tc = (int)std::min<int64_t>(b.ntile, 2 * kPersist);
   switch (mode) {
@@ -1207,6 +1208,7 @@ void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipSt
       return;
     case EPI_NORMAL:
       hipLaunchKernelGGL(conv_kernel<EPI_NORMAL>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      if (!reduce) return;
       hipLaunchKernelGGL(normal_kernel, dim3(tc), dim3(256), 0, stream, b, s);
       break;
     case EPI_RPP:
@@ -1224,7 +1226,7 @@ void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipSt
 int run_stage(const apx_batch& b, const Scratch& s, int stage, int mode, const double* rep_override,
               float* out, hipStream_t stream, int parts = 3) {
   if (parts & 1) render_part(b, s, stage, rep_override, stream);
-  if (parts & 2) conv_part(b, s, mode, out, stream);
+  if (parts & 2) conv_part(b, s, mode, out, stream, stage != ST_ALL && stage != ST_ALL_JAC);
   return check_launch("batch stage");
 }
 
@@ -1294,6 +1296,16 @@ int batch_render(const apx_batch* b, const double* rep, float* out, int32_t part
   if (parts < 1 || parts > 3) return set_error(APX_E_INVALID, "batch_render: parts must be 1, 2 or 3");
   const Scratch s = carve(*b);
   return run_stage(*b, s, ST_ALL, EPI_OUT, rep, out, stream, parts);
+}
+
+int batch_jacobian(const apx_batch* b, const double* rep, float* out, int32_t parts,
+                   hipStream_t stream) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!rep || !out) return set_error(APX_E_INVALID, "batch_jacobian: null pointer");
+  if (parts < 1 || parts > 3) return set_error(APX_E_INVALID, "batch_jacobian: parts must be 1, 2 or 3");
+  const Scratch s = carve(*b);
+  return run_stage(*b, s, ST_ALL_JAC, EPI_NORMAL, rep, out, stream, parts);
 }
 
 int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream) {

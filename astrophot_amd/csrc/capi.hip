@@ -110,6 +110,11 @@ int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int3
   return apx::batch_render(batch, rep, out, parts, static_cast<hipStream_t>(stream));
 }
 
+int apx_batch_jacobian(const apx_batch* batch, const double* rep, float* out, int32_t parts,
+                       void* stream) {
+  return apx::batch_jacobian(batch, rep, out, parts, static_cast<hipStream_t>(stream));
+}
+
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream) {
   return apx::batch_start(batch, rep0, static_cast<hipStream_t>(stream));
 }

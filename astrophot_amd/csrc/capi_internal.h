@@ -45,6 +45,8 @@ int64_t batch_state_bytes();
 int64_t batch_scratch_bytes(const apx_batch* b);
 int batch_render(const apx_batch* b, const double* rep, float* out, int32_t parts,
                  hipStream_t stream);
+int batch_jacobian(const apx_batch* b, const double* rep, float* out, int32_t parts,
+                 hipStream_t stream);
 int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream);
 int batch_tick(const apx_batch* b, hipStream_t stream);
 int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,

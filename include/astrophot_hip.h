@@ -233,7 +233,8 @@ typedef struct apx_batch {
   const float* weight;         /* npix, 1/variance */
   const float* psf;            /* sum of ph * pw */
   float* model0;               /* npix: model at the current state */
-  float* model1;               /* npix: probe / trial model */
+  float* model1;               /* npix: spare model buffer (the LM ticks reduce their
+                                  probe and trial renders in place, unstored) */
   float* jac;                  /* npix x APX_BATCH_NP, pixel-major */
   float* planes;               /* 6 x nwork */
   float* taps;                 /* ntap */
@@ -251,6 +252,15 @@ int64_t apx_batch_scratch_bytes(const apx_batch* batch);
  * a previous parts=1 call, 3 = both. */
 int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int32_t parts,
                      void* stream);
+
+/* Render every cutout at rep into out (npix) and its Jacobian with respect
+ * to the 7 representation-space parameters into batch->jac (npix x 7,
+ * pixel-major): the batched counterpart of Component_Model.jacobian(
+ * as_representation=True) (models/_model_methods.py:260-347), the columns of
+ * the J the LM's Jacobian stage forms (fit/lm.py:256-257).  parts as in
+ * apx_batch_render.  Leaves the LM state of the fits untouched. */
+int apx_batch_jacobian(const apx_batch* batch, const double* rep, float* out, int32_t parts,
+                       void* stream);
 
 /* Start the fits from rep0 (device, ngal x 7). */
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream);

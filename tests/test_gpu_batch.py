@@ -77,6 +77,30 @@ def test_batch_render_matches_oracle():
         assert float(((got - ref).abs() / ref)[bright].max()) < 1e-5, k
 
 
+def test_batch_jacobian_matches_single_model():
+    """apx_batch_jacobian: every cutout's model and its 7 representation-space
+    Jacobian columns equal the single-model jacobian(as_representation=True)
+    within the Jacobian parity bar (2e-4 of each column's max)."""
+    truths, shapes, rng = make_cutouts(4, seed=3)
+    cuts = [Cutout(np.zeros((s, s)), np.ones((s, s)), psf) for s, psf in shapes]
+    bf = BatchFit(cuts)
+    rep = values_to_rep(np.stack(truths))
+    model, J = bf.jacobian(rep)
+    imgs = bf.split(model)
+    for k, ((s, psf), v, g) in enumerate(zip(shapes, truths, bf.gal_host)):
+        tgt = ap.image.Target_Image(data=np.zeros((s, s)), pixelscale=1.0, psf=ap.image.PSF_Image(data=psf, pixelscale=1.0))
+        p = {"center": [v[0], v[1]], "q": v[2], "PA": v[3], "n": v[4], "Re": v[5], "Ie": v[6]}
+        m = ap.models.AstroPhot_Model(name=f"j{k}", model_type="sersic galaxy model", target=tgt, parameters=p,
+                                      psf_mode="full")
+        ref = m().data
+        assert float((imgs[k] - ref).abs().max() / ref.abs().max()) < 1e-6, k
+        rj = m.jacobian(as_representation=True).flatten("data")
+        got = J[g.pix_off:g.pix_off + s * s]
+        for c in range(7):
+            err = float((got[:, c].double() - rj[:, c].double()).abs().max() / rj[:, c].abs().max())
+            assert err < 2e-4, (k, c, err)
+
+
 def _problem(n, seed):
     truths, shapes, rng = make_cutouts(n, seed)
     cuts, starts = [], []

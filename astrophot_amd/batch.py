@@ -30,7 +30,7 @@ from .errors import SpecificationConflict
 NP = 7
 PARAM_NAMES = ("center_x", "center_y", "q", "PA", "n", "Re", "Ie")
 STATUS = {0: "running", 1: "success", 2: "success by immobility. Convergence not guaranteed",
-          3: "fail. Could not find step to improve Chi^2", 4: "fail. Maximum iterations"}
+          3: "fail. Could not find step to improve Chi^2", 4: "fail. Maximum iterations", 5: "skipped"}
 
 
 class Galaxy(ctypes.Structure):
@@ -64,6 +64,8 @@ SIGNATURES = {
                                         ctypes.c_void_p]),
     "apx_batch_jacobian": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_int32, ctypes.c_void_p]),
+    "apx_batch_moment_init": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
+    "apx_sersic_light_fraction": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
     "apx_batch_start": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
     "apx_batch_tick": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p]),
     "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 6),
@@ -195,6 +197,13 @@ class BatchFit:
         L.check(self.lib.apx_batch_jacobian(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), parts,
                                             L.stream()))
         return out, self.jac.view(self.npix, NP)
+
+    def moment_init(self) -> torch.Tensor:
+        """Moment-based starting parameters of every cutout from its data
+        (ngal x 7, representation space, float64 on the device)."""
+        rep = torch.empty((len(self.gal_host), NP), dtype=torch.float64, device=self.data.device)
+        L.check(self.lib.apx_batch_moment_init(ctypes.byref(self.b), rep.data_ptr(), L.stream()))
+        return rep
 
     def conv_flops(self) -> float:
         """Algorithmic flops of one value convolution of every cutout:

@@ -1,0 +1,301 @@
+"""Learned initialiser: a small CNN that proposes Sersic starting parameters.
+
+The network maps a 64 x 64 cutout, normalised as asinh(x / sigma) with sigma
+the cutout's noise level, to the 7 Sersic parameters LM starts from.
+
+Layers:
+  4 convolutions 3x3 / stride 2 / pad 1, ReLU: 1 -> 32 -> 32 -> 64 -> 64
+      channels, 64^2 -> 4^2 pixels;
+  fc 1024 -> 128, ReLU;
+  fc 128 -> 8.
+
+The 8 outputs encode the 7 parameters: centre offset, q, (cos 2PA, sin 2PA),
+ln n, ln Re, and Ie relative to the noise.
+
+Forward, backward and Adam run on this package's kernels (csrc/cnn.hip): a
+BF16 MFMA GEMM with float32 accumulation, im2col / col2im, bias gradients and
+the loss.  Torch only holds the buffers.  The reference has no counterpart:
+AstroPhot starts LM from its initialise chain, which stays the fallback here
+(BatchFit.moment_init).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+SIDE = 64
+CONV = [(1, 32), (32, 32), (32, 64), (64, 64)]
+FC = [(4 * 4 * 64, 128), (128, 8)]
+NOUT = 8
+
+_p, _i32, _i64, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+SIGNATURES = {
+    "apx_gemm_bf16": (ctypes.c_int, [_i32, _i32, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _i32,
+                                     _p, _i32, _i32, _p, _i64, _p]),
+    "apx_mse": (ctypes.c_int, [_p, _p, _p, _i32, _i32, _p, _p, _p]),
+    "apx_im2col_s2": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _i32, _p, _p]),
+    "apx_col2im_s2": (ctypes.c_int, [_p, _i32, _i32, _i32, _i32, _i32, _p, _p]),
+    "apx_cast_gate_bf16": (ctypes.c_int, [_p, _p, _i64, _p, _p, _p]),
+    "apx_colsum": (ctypes.c_int, [_p, _i32, _i32, _p, _p]),
+    "apx_adam": (ctypes.c_int, [_p, _p, _p, _p, _i64, _f32, _f32, _f32, _f32, _i32, _p, _p]),
+    "apx_asinh_input": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p]),
+}
+L.SIGNATURES.update(SIGNATURES)
+
+
+def kpad(k: int) -> int:
+    return (k + 31) // 32 * 32
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else t.data_ptr()
+
+
+# ---------------------------------------------------------------------------
+# primitives (thin wrappers over the C ABI)
+def gemm(A, sam, sak, B, sbk, sbn, C, M, N, K, ldc=None, bias=None, relu=False, Cb=None, accumulate=False,
+         ws=None):
+    """C = A B (+ bias, ReLU) with A(m,k) = A[m sam + k sak], B(k,n) = B[k sbk + n sbn]."""
+    lib = L.load()
+    ldc = N if ldc is None else ldc
+    tiles = ((M + 63) // 64) * ((N + 63) // 64)
+    splits = 1
+    if K >= 4096 and tiles < 128 and ws is not None:
+        splits = int(min(max(1, K // 2048), 256, ws.numel() // max(M * N, 1)))
+    L.check(lib.apx_gemm_bf16(M, N, K, A.data_ptr(), sam, sak, B.data_ptr(), sbk, sbn, C.data_ptr(), ldc,
+                              _ptr(bias), int(relu), _ptr(Cb), int(accumulate), splits,
+                              _ptr(ws) if splits > 1 else None, ws.numel() if ws is not None else 0,
+                              L.stream()))
+
+
+def cast_bf16(src, dst, mask=None, dstf=None):
+    L.check(L.load().apx_cast_gate_bf16(src.data_ptr(), _ptr(mask), src.numel(), dst.data_ptr(), _ptr(dstf),
+                                        L.stream()))
+
+
+# ---------------------------------------------------------------------------
+# target encoding (value space, cutout pixel coordinates with the corner at 0)
+ENC_WEIGHTS = np.ones(NOUT, dtype=np.float32)
+
+
+def encode(values: np.ndarray, sigma: np.ndarray, side: int = SIDE) -> np.ndarray:
+    """[cx, cy, q, PA, n, Re, Ie] (cutout-relative plane coordinates) -> the 8 network targets."""
+    v = np.asarray(values, dtype=np.float64)
+    t = np.empty(v.shape[:-1] + (NOUT,))
+    t[..., 0] = (v[..., 0] - side / 2) / 2.0
+    t[..., 1] = (v[..., 1] - side / 2) / 2.0
+    t[..., 2] = (v[..., 2] - 0.6) / 0.25
+    t[..., 3] = np.cos(2 * v[..., 3])
+    t[..., 4] = np.sin(2 * v[..., 3])
+    t[..., 5] = (np.log(v[..., 4]) - math.log(1.7)) / 0.7
+    t[..., 6] = (np.log(v[..., 5]) - math.log(6.0)) / 0.7
+    t[..., 7] = (v[..., 6] - np.log10(sigma) - 1.2) / 0.8
+    return t.astype(np.float32)
+
+
+def decode(t: np.ndarray, sigma: np.ndarray, side: int = SIDE) -> np.ndarray:
+    """Network outputs -> [cx, cy, q, PA, n, Re, Ie] inside the parameter limits."""
+    t = np.asarray(t, dtype=np.float64)
+    v = np.empty(t.shape[:-1] + (7,))
+    v[..., 0] = side / 2 + 2.0 * t[..., 0]
+    v[..., 1] = side / 2 + 2.0 * t[..., 1]
+    v[..., 2] = np.clip(0.6 + 0.25 * t[..., 2], 0.05, 0.98)
+    v[..., 3] = np.mod(0.5 * np.arctan2(t[..., 4], t[..., 3]), np.pi)
+    v[..., 4] = np.clip(np.exp(math.log(1.7) + 0.7 * t[..., 5]), 0.4, 7.5)
+    v[..., 5] = np.clip(np.exp(math.log(6.0) + 0.7 * t[..., 6]), 0.3, 4 * side)
+    v[..., 6] = np.log10(sigma) + 1.2 + 0.8 * t[..., 7]
+    return v
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class Layer:
+    kind: str       # "conv" or "fc"
+    cin: int
+    cout: int
+    K: int          # GEMM depth (padded for convolutions)
+    w: torch.Tensor     # (K, cout) float32 master
+    b: torch.Tensor     # (cout,)
+    wb: torch.Tensor    # bf16 copy of w
+    gw: torch.Tensor
+    gb: torch.Tensor
+    mw: torch.Tensor
+    vw: torch.Tensor
+    mb: torch.Tensor
+    vb: torch.Tensor
+
+
+class InitNet:
+    """The network, its Adam state and per-batch-size activation buffers."""
+
+    def __init__(self, seed: int = 0, device: str = "cuda"):
+        self.device = device
+        gen = torch.Generator().manual_seed(seed)
+        self.layers: List[Layer] = []
+        for cin, cout in CONV:
+            self.layers.append(self._layer("conv", cin, cout, kpad(9 * cin), 9 * cin, gen))
+        for i, (cin, cout) in enumerate(FC):
+            self.layers.append(self._layer("fc", cin, cout, cin, cin, gen, gain=0.5 if i == len(FC) - 1 else 2.0))
+        self.step_count = 0
+        self._bufs: Dict[int, dict] = {}
+
+    def _layer(self, kind, cin, cout, K, fan_in, gen, gain=2.0):
+        w = torch.zeros((K, cout), dtype=torch.float32)
+        w[:fan_in] = torch.randn((fan_in, cout), generator=gen) * math.sqrt(gain / fan_in)
+        d = dict(device=self.device)
+        w = w.to(**d)
+        z = lambda *s: torch.zeros(s, dtype=torch.float32, **d)  # noqa: E731
+        lay = Layer(kind, cin, cout, K, w, z(cout), torch.empty((K, cout), dtype=torch.bfloat16, **d),
+                    z(K, cout), z(cout), z(K, cout), z(K, cout), z(cout), z(cout))
+        cast_bf16(lay.w, lay.wb)
+        return lay
+
+    # -- state -------------------------------------------------------------
+    def state_dict(self) -> Dict[str, np.ndarray]:
+        out = {}
+        for i, lay in enumerate(self.layers):
+            out[f"w{i}"] = lay.w.cpu().numpy()
+            out[f"b{i}"] = lay.b.cpu().numpy()
+        return out
+
+    def load_state_dict(self, st) -> "InitNet":
+        for i, lay in enumerate(self.layers):
+            lay.w.copy_(torch.as_tensor(np.asarray(st[f"w{i}"]), dtype=torch.float32))
+            lay.b.copy_(torch.as_tensor(np.asarray(st[f"b{i}"]), dtype=torch.float32))
+            cast_bf16(lay.w, lay.wb)
+        return self
+
+    def save(self, path):
+        np.savez(path, **self.state_dict())
+
+    @classmethod
+    def load(cls, path, device="cuda") -> "InitNet":
+        with np.load(path, allow_pickle=False) as f:
+            return cls(device=device).load_state_dict({k: f[k] for k in f.files})
+
+    # -- buffers -----------------------------------------------------------
+    def _buffers(self, B):
+        if B in self._bufs:
+            return self._bufs[B]
+        d = dict(device=self.device)
+        bf = lambda *s: torch.empty(s, dtype=torch.bfloat16, **d)  # noqa: E731
+        f32 = lambda *s: torch.empty(s, dtype=torch.float32, **d)  # noqa: E731
+        buf = {"x": bf(B * SIDE * SIDE), "col": [], "y": [], "yb": [], "dims": []}
+        H = SIDE
+        for (cin, cout), lay in zip(CONV, self.layers):
+            Ho = (H + 1) // 2
+            M = B * Ho * Ho
+            buf["dims"].append((H, cin, cout, M, lay.K))
+            buf["col"].append(bf(M, lay.K))
+            buf["y"].append(f32(M, cout))
+            buf["yb"].append(bf(M, cout))
+            H = Ho
+        buf["h"], buf["hb"] = f32(B, FC[0][1]), bf(B, FC[0][1])
+        buf["out"] = f32(B, NOUT)
+        big = max(max(M * K for (_, _, _, M, K) in buf["dims"]), B * FC[0][0])
+        buf["g32"], buf["g32b"] = f32(big), f32(big)
+        buf["gbf"], buf["gbf2"] = bf(big), bf(big)
+        buf["ws"] = f32(1 << 22)
+        buf["dout"], buf["loss"] = f32(B, NOUT), f32(1)
+        self._bufs[B] = buf
+        return buf
+
+    # -- passes ------------------------------------------------------------
+    def input(self, images: torch.Tensor, sigma: torch.Tensor) -> torch.Tensor:
+        """images (B, 64, 64) float32 on the device, sigma (B,) -> the bf16 network input."""
+        B = images.shape[0]
+        buf = self._buffers(B)
+        L.check(L.load().apx_asinh_input(images.contiguous().data_ptr(), sigma.contiguous().data_ptr(), B,
+                                         SIDE * SIDE, buf["x"].data_ptr(), L.stream()))
+        return buf["x"]
+
+    def forward(self, x: torch.Tensor, B: int) -> torch.Tensor:
+        """x: bf16 input of B cutouts (NHWC, C = 1) -> (B, 8) float32 outputs."""
+        lib = L.load()
+        buf = self._buffers(B)
+        cur = x
+        for i, (H, cin, cout, M, K) in enumerate(buf["dims"]):
+            lay = self.layers[i]
+            L.check(lib.apx_im2col_s2(cur.data_ptr(), B, H, H, cin, K, buf["col"][i].data_ptr(), L.stream()))
+            gemm(buf["col"][i], K, 1, lay.wb, cout, 1, buf["y"][i], M, cout, K, bias=lay.b, relu=True,
+                 Cb=buf["yb"][i])
+            cur = buf["yb"][i]
+        f1, f2 = self.layers[4], self.layers[5]
+        gemm(cur, f1.cin, 1, f1.wb, f1.cout, 1, buf["h"], B, f1.cout, f1.cin, bias=f1.b, relu=True, Cb=buf["hb"])
+        gemm(buf["hb"], f2.cin, 1, f2.wb, f2.cout, 1, buf["out"], B, f2.cout, f2.cin, bias=f2.b)
+        return buf["out"]
+
+    def backward(self, B: int, dout: torch.Tensor):
+        """Gradients of every weight and bias from dL/d(out) (B, 8) float32;
+        forward() must have run on this batch."""
+        lib = L.load()
+        buf = self._buffers(B)
+        ws = buf["ws"]
+        f1, f2 = self.layers[4], self.layers[5]
+        db = buf["gbf"][:dout.numel()]
+        cast_bf16(dout, db)
+        # fc2
+        gemm(buf["hb"], 1, f2.cin, db, f2.cout, 1, f2.gw, f2.cin, f2.cout, B, ws=ws)
+        L.check(lib.apx_colsum(dout.data_ptr(), B, f2.cout, f2.gb.data_ptr(), L.stream()))
+        dh = buf["g32"][:B * f2.cin]
+        gemm(db, f2.cout, 1, f2.wb, 1, f2.cout, dh, B, f2.cin, f2.cout)
+        dz = buf["gbf2"][:B * f2.cin]
+        dzf = buf["g32b"][:B * f2.cin]
+        cast_bf16(dh, dz, mask=buf["h"], dstf=dzf)
+        # fc1
+        y4b = buf["yb"][3]
+        gemm(y4b, 1, f1.cin, dz, f1.cout, 1, f1.gw, f1.cin, f1.cout, B, ws=ws)
+        L.check(lib.apx_colsum(dzf.data_ptr(), B, f1.cout, f1.gb.data_ptr(), L.stream()))
+        dy = buf["g32"][:B * f1.cin]
+        gemm(dz, f1.cout, 1, f1.wb, 1, f1.cout, dy, B, f1.cin, f1.cout)
+        # convolutions, last to first: dz_l = dy_l * (y_l > 0).  dy of the
+        # last convolution is in g32; col2im leaves the others in g32b, where
+        # the (element-wise) gate then writes its float32 copy in place.
+        for i in range(len(CONV) - 1, -1, -1):
+            H, cin, cout, M, K = buf["dims"][i]
+            lay = self.layers[i]
+            dzb = buf["gbf"][:M * cout]
+            dzf = buf["g32b"][:M * cout]
+            cast_bf16(dy, dzb, mask=buf["y"][i], dstf=dzf)
+            gemm(buf["col"][i], 1, K, dzb, cout, 1, lay.gw, K, cout, M, ws=ws)
+            L.check(lib.apx_colsum(dzf.data_ptr(), M, cout, lay.gb.data_ptr(), L.stream()))
+            if i == 0:
+                break
+            dcol = buf["g32"][:M * K]
+            gemm(dzb, cout, 1, lay.wb, 1, cout, dcol, M, K, cout)
+            dy = buf["g32b"][:B * H * H * cin]
+            L.check(lib.apx_col2im_s2(dcol.data_ptr(), B, H, H, cin, K, dy.data_ptr(), L.stream()))
+
+    def loss_grad(self, out: torch.Tensor, target: torch.Tensor, weights: torch.Tensor, B: int):
+        buf = self._buffers(B)
+        L.check(L.load().apx_mse(out.data_ptr(), target.data_ptr(), weights.data_ptr(), B, NOUT,
+                                 buf["dout"].data_ptr(), buf["loss"].data_ptr(), L.stream()))
+        return buf["dout"], buf["loss"]
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        lib = L.load()
+        self.step_count += 1
+        for lay in self.layers:
+            L.check(lib.apx_adam(lay.w.data_ptr(), lay.gw.data_ptr(), lay.mw.data_ptr(), lay.vw.data_ptr(),
+                                 lay.w.numel(), lr, beta1, beta2, eps, self.step_count, lay.wb.data_ptr(),
+                                 L.stream()))
+            L.check(lib.apx_adam(lay.b.data_ptr(), lay.gb.data_ptr(), lay.mb.data_ptr(), lay.vb.data_ptr(),
+                                 lay.b.numel(), lr, beta1, beta2, eps, self.step_count, None, L.stream()))
+
+    # -- inference ---------------------------------------------------------
+    def predict(self, images: torch.Tensor, sigma: torch.Tensor, batch: int = 4096) -> np.ndarray:
+        """[cx, cy, q, PA, n, Re, Ie] (cutout-relative) for every cutout."""
+        outs = []
+        for s in range(0, images.shape[0], batch):
+            im, sg = images[s:s + batch], sigma[s:s + batch]
+            B = im.shape[0]
+            outs.append(self.forward(self.input(im, sg), B).cpu().numpy().copy())
+        return decode(np.concatenate(outs), sigma.cpu().numpy())

@@ -39,7 +39,7 @@ constexpr int kNChan = 6;  // value + q, PA, n, Re, Ie partials
 enum : int32_t { PH_INIT = 0, PH_NEEDJ = 1, PH_SUB = 2, PH_DONE = 3 };
 enum : int32_t { ST_JAC = 0, ST_PROBE = 1, ST_TRIAL = 2, ST_ALL = 3, ST_ALL_JAC = 4 };
 enum : int32_t { DIR_NONE = 0, DIR_BETTER = 1, DIR_WORSE = 2 };
-enum : int32_t { S_RUN = 0, S_SUCCESS = 1, S_IMMOBILE = 2, S_NOSTEP = 3, S_MAXITER = 4 };
+enum : int32_t { S_RUN = 0, S_SUCCESS = 1, S_IMMOBILE = 2, S_NOSTEP = 3, S_MAXITER = 4, S_SKIPPED = 5 };
 
 struct GalState {
   double x[8], ev[8];
@@ -949,8 +949,10 @@ __global__ void start_kernel(apx_batch b, const double* rep0) {
   st->iter = 0;
   st->res_chi2 = INFINITY;
   for (int i = 0; i < NP; ++i) st->res_x[i] = st->x[i];
-  st->phase = PH_INIT;
-  st->status = S_RUN;
+  bool finite = true;
+  for (int i = 0; i < NP; ++i) finite = finite && isfinite(st->x[i]);
+  st->phase = finite ? PH_INIT : PH_DONE;  // a non-finite start is not run
+  st->status = finite ? S_RUN : S_SKIPPED;
   st->ndf = fmax(1.0, (double)G.nx * G.ny - NP);
 }
 

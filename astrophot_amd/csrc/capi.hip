@@ -115,6 +115,14 @@ int apx_batch_jacobian(const apx_batch* batch, const double* rep, float* out, in
   return apx::batch_jacobian(batch, rep, out, parts, static_cast<hipStream_t>(stream));
 }
 
+int apx_batch_moment_init(const apx_batch* batch, double* rep, void* stream) {
+  return apx::batch_moment_init(batch, rep, static_cast<hipStream_t>(stream));
+}
+
+double apx_sersic_light_fraction(double n, double r_over_re) {
+  return apx::sersic_light_fraction(n, r_over_re);
+}
+
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream) {
   return apx::batch_start(batch, rep0, static_cast<hipStream_t>(stream));
 }
@@ -127,6 +135,50 @@ int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t
                       int32_t* iterations, int32_t* done, void* stream) {
   return apx::batch_results(batch, rep, chi2, status, iterations, done,
                             static_cast<hipStream_t>(stream));
+}
+
+
+/* ---- learned initialiser: BF16 MFMA network primitives ---------------- */
+int apx_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t* A, int64_t sam, int64_t sak,
+                  const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc,
+                  const float* bias, int32_t relu, uint16_t* Cb, int32_t accumulate, int32_t splits,
+                  float* ws, int64_t ws_floats, void* stream) {
+  return apx::gemm_bf16(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, relu, Cb, accumulate, splits,
+                        ws, ws_floats, static_cast<hipStream_t>(stream));
+}
+
+int apx_mse(const float* y, const float* t, const float* w, int32_t B, int32_t N, float* dy,
+            float* loss, void* stream) {
+  return apx::mse(y, t, w, B, N, dy, loss, static_cast<hipStream_t>(stream));
+}
+
+int apx_im2col_s2(const uint16_t* X, int32_t B, int32_t H, int32_t W, int32_t C, int32_t Kp,
+                  uint16_t* col, void* stream) {
+  return apx::im2col_s2(X, B, H, W, C, Kp, col, static_cast<hipStream_t>(stream));
+}
+
+int apx_col2im_s2(const float* dcol, int32_t B, int32_t H, int32_t W, int32_t C, int32_t Kp,
+                  float* dX, void* stream) {
+  return apx::col2im_s2(dcol, B, H, W, C, Kp, dX, static_cast<hipStream_t>(stream));
+}
+
+int apx_cast_gate_bf16(const float* src, const float* mask, int64_t n, uint16_t* dst, float* dstf,
+                       void* stream) {
+  return apx::cast_gate(src, mask, n, dst, dstf, static_cast<hipStream_t>(stream));
+}
+
+int apx_colsum(const float* X, int32_t M, int32_t N, float* out, void* stream) {
+  return apx::colsum(X, M, N, out, static_cast<hipStream_t>(stream));
+}
+
+int apx_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+             float beta2, float eps, int32_t step, uint16_t* w_bf16, void* stream) {
+  return apx::adam(w, g, m, v, n, lr, beta1, beta2, eps, step, w_bf16, static_cast<hipStream_t>(stream));
+}
+
+int apx_asinh_input(const float* x, const float* sigma, int32_t B, int32_t npx, uint16_t* out,
+                    void* stream) {
+  return apx::asinh_input(x, sigma, B, npx, out, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

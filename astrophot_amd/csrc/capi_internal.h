@@ -49,6 +49,23 @@ int batch_jacobian(const apx_batch* b, const double* rep, float* out, int32_t pa
                  hipStream_t stream);
 int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream);
 int batch_tick(const apx_batch* b, hipStream_t stream);
+int batch_moment_init(const apx_batch* b, double* rep, hipStream_t stream);
+double sersic_light_fraction(double n, double r_over_re);
+int gemm_bf16(int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+              int64_t sbk, int64_t sbn, float* C, int64_t ldc, const float* bias, int relu,
+              uint16_t* Cb, int accumulate, int splits, float* ws, int64_t ws_floats,
+              hipStream_t stream);
+int mse(const float* y, const float* t, const float* w, int Bn, int N, float* dy, float* loss,
+        hipStream_t stream);
+int im2col_s2(const uint16_t* X, int Bn, int H, int W, int Cin, int Kp, uint16_t* col,
+              hipStream_t stream);
+int col2im_s2(const float* dcol, int Bn, int H, int W, int Cin, int Kp, float* dX, hipStream_t stream);
+int cast_gate(const float* src, const float* mask, int64_t n, uint16_t* dst, float* dstf,
+              hipStream_t stream);
+int colsum(const float* X, int M, int N, float* out, hipStream_t stream);
+int adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+         float eps, int step, uint16_t* wb, hipStream_t stream);
+int asinh_input(const float* x, const float* sigma, int Bn, int npx, uint16_t* out, hipStream_t stream);
 int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,
                   int32_t* done, hipStream_t stream);
 

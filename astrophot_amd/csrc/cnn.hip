@@ -1,0 +1,331 @@
+// The learned initialiser's network on gfx950: a BF16 MFMA GEMM (float32
+// accumulate) with a bias / ReLU epilogue, and the small kernels around it
+// that make convolutions, their gradients and an Adam step out of GEMMs.
+//
+// Layout: activations NHWC (row = one output pixel of one cutout, column =
+// channel), so a 3x3 / stride-2 / pad-1 convolution is
+//   Y[M = B*Ho*Wo, Cout] = im2col(X)[M, Kp] * W[Kp, Cout]      (K = 9 Cin)
+// with K ordered (ky, kx, ci) and zero-padded to a multiple of 32; its
+// gradients are
+//   dW = im2col(X)^T dZ,   dcol = dZ W^T,   dX = col2im(dcol)  (a gather).
+// Bf16 values are stored as uint16 (round to nearest even from float32).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "capi_internal.h"
+
+namespace apx {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------------------
+// C[M, N] = A[M, K] B[K, N] (+ bias[N], ReLU), bf16 in, float32 accumulate.
+// Element (m, k) of A at A[m * sam + k * sak]; (k, n) of B at B[k * sbk + n * sbn]
+// (so a transposed operand is a stride swap).  Workgroup tile 64 x 64,
+// k-step 32, 4 waves of 32 x 32 (2 x 2 MFMA 16x16x32 blocks).  MFMA lane
+// layout: A row l % 16, k 8 (l / 16) .. + 7; B column l % 16, same k;
+// D[4 (l / 16) + r][l % 16] in acc[r].
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // LDS row pitch (bf16), padded
+
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const uint16_t* A,
+                                                        int64_t sam, int64_t sak, const uint16_t* B,
+                                                        int64_t sbk, int64_t sbn, float* C,
+                                                        int64_t ldc, const float* bias, int relu,
+                                                        uint16_t* Cb, int accumulate, float* P,
+                                                        int kchunk) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDK];  // Bs[n][k]
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  v4f acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (v4f){0.0f, 0.0f, 0.0f, 0.0f};
+  // staging: thread t loads 8 consecutive k of one row (A) / one column (B)
+  const int sr = threadIdx.x >> 2, sk = (threadIdx.x & 3) * 8;
+  const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    __syncthreads();
+    {
+      const int m = m0 + sr;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + sk + q;
+        As[sr * LDK + sk + q] = (m < M && k < kend) ? A[(int64_t)m * sam + (int64_t)k * sak] : (uint16_t)0;
+      }
+      const int n = n0 + sr;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + sk + q;
+        Bs[sr * LDK + sk + q] = (n < N && k < kend) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    const int kq = (lane >> 4) * 8;
+    v8bf a[2], bb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int r = wm * 32 + i * 16 + (lane & 15);
+      a[i] = *reinterpret_cast<const v8bf*>(&As[r * LDK + kq]);
+      const int c = wn * 32 + i * 16 + (lane & 15);
+      bb[i] = *reinterpret_cast<const v8bf*>(&Bs[c * LDK + kq]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+  }
+  // epilogue
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + r;
+        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (m >= M || n >= N) continue;
+        float v = acc[i][j][r];
+        if (P) {  // split-K partial, summed by splitk_sum_kernel
+          P[((int64_t)blockIdx.z * M + m) * N + n] = v;
+          continue;
+        }
+        if (accumulate) v += C[(int64_t)m * ldc + n];
+        if (bias) v += bias[n];
+        if (relu) v = fmaxf(v, 0.0f);
+        C[(int64_t)m * ldc + n] = v;
+        if (Cb) Cb[(int64_t)m * ldc + n] = to_bf16(v);
+      }
+}
+
+// C = sum over the split-K partials (in split order) (+ C, bias, ReLU)
+__global__ __launch_bounds__(256) void splitk_sum_kernel(const float* P, int splits, int M, int N,
+                                                         float* C, int64_t ldc, const float* bias,
+                                                         int relu, uint16_t* Cb, int accumulate) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int m = (int)(e / N), n = (int)(e % N);
+    float v = 0.0f;
+    for (int z = 0; z < splits; ++z) v += P[(int64_t)z * total + e];
+    if (accumulate) v += C[(int64_t)m * ldc + n];
+    if (bias) v += bias[n];
+    if (relu) v = fmaxf(v, 0.0f);
+    C[(int64_t)m * ldc + n] = v;
+    if (Cb) Cb[(int64_t)m * ldc + n] = to_bf16(v);
+  }
+}
+
+// weighted mean-squared error over B x N outputs: loss[0] = sum_bj w_j (y - t)^2 / B,
+// dy = 2 w_j (y - t) / B (one workgroup, fixed summation order)
+__global__ __launch_bounds__(256) void mse_kernel(const float* y, const float* t, const float* wj, int Bn,
+                                                  int N, float* dy, float* loss) {
+  __shared__ float red[4];
+  float s = 0.0f;
+  for (int e = threadIdx.x; e < Bn * N; e += 256) {
+    const float d = y[e] - t[e], w = wj[e % N];
+    dy[e] = 2.0f * w * d / (float)Bn;
+    s += w * d * d;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) loss[0] = (red[0] + red[1] + red[2] + red[3]) / (float)Bn;
+}
+
+// im2col of a 3x3 / stride-2 / pad-1 convolution: X (B, H, W, Cin) bf16 ->
+// col (B * Ho * Wo, Kp) bf16, column (ky * 3 + kx) * Cin + ci, zero beyond K
+__global__ __launch_bounds__(256) void im2col_kernel(const uint16_t* X, int Bn, int H, int W, int Cin,
+                                                     int Kp, uint16_t* col) {
+  const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+  const int64_t total = (int64_t)Bn * Ho * Wo * Kp;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int k = (int)(e % Kp);
+    const int64_t m = e / Kp;
+    uint16_t v = 0;
+    if (k < 9 * Cin) {
+      const int ci = k % Cin, t = k / Cin, ky = t / 3, kx = t % 3;
+      const int ox = (int)(m % Wo), oy = (int)((m / Wo) % Ho), b = (int)(m / ((int64_t)Wo * Ho));
+      const int iy = 2 * oy + ky - 1, ix = 2 * ox + kx - 1;
+      if (iy >= 0 && iy < H && ix >= 0 && ix < W) v = X[(((int64_t)b * H + iy) * W + ix) * Cin + ci];
+    }
+    col[e] = v;
+  }
+}
+
+// dX (B, H, W, Cin) float32 = col2im(dcol (B * Ho * Wo, Kp)): each input
+// element gathers the (oy, ky) x (ox, kx) pairs that read it
+__global__ __launch_bounds__(256) void col2im_kernel(const float* dcol, int Bn, int H, int W, int Cin,
+                                                     int Kp, float* dX) {
+  const int Ho = (H + 1) / 2, Wo = (W + 1) / 2;
+  const int64_t total = (int64_t)Bn * H * W * Cin;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int ci = (int)(e % Cin);
+    const int ix = (int)((e / Cin) % W), iy = (int)((e / ((int64_t)Cin * W)) % H);
+    const int b = (int)(e / ((int64_t)Cin * W * H));
+    float s = 0.0f;
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky) {
+      const int ty = iy + 1 - ky;  // = 2 oy
+      if (ty < 0 || (ty & 1) || ty / 2 >= Ho) continue;
+#pragma unroll
+      for (int kx = 0; kx < 3; ++kx) {
+        const int tx = ix + 1 - kx;
+        if (tx < 0 || (tx & 1) || tx / 2 >= Wo) continue;
+        const int64_t m = ((int64_t)b * Ho + ty / 2) * Wo + tx / 2;
+        s += dcol[m * Kp + (ky * 3 + kx) * Cin + ci];
+      }
+    }
+    dX[e] = s;
+  }
+}
+
+// dst = bf16(src * (mask > 0)) (mask null: no ReLU gate)
+__global__ __launch_bounds__(256) void cast_gate_kernel(const float* src, const float* mask, int64_t n,
+                                                        uint16_t* dst, float* dstf) {
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    float v = src[e];
+    if (mask && !(mask[e] > 0.0f)) v = 0.0f;
+    if (dstf) dstf[e] = v;
+    dst[e] = to_bf16(v);
+  }
+}
+
+// column sums of X (M, N) float32 into out (N): the bias gradient
+__global__ __launch_bounds__(256) void colsum_kernel(const float* X, int M, int N, float* out) {
+  __shared__ float red[4][64];
+  const int n = blockIdx.x * 64 + (threadIdx.x & 63), part = threadIdx.x >> 6;
+  float s = 0.0f;
+  if (n < N)
+    for (int m = part; m < M; m += 4) s += X[(int64_t)m * N + n];
+  red[part][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (part == 0 && n < N) out[n] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// Adam (Kingma & Ba 2015) on float32 master weights, refreshing their bf16 copy
+__global__ __launch_bounds__(256) void adam_kernel(float* w, const float* g, float* m, float* v, int64_t n,
+                                                   float lr, float b1, float b2, float eps, float c1,
+                                                   float c2, uint16_t* wb) {
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const float gg = g[e];
+    const float mm = b1 * m[e] + (1.0f - b1) * gg;
+    const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+    m[e] = mm;
+    v[e] = vv;
+    const float ww = w[e] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    w[e] = ww;
+    if (wb) wb[e] = to_bf16(ww);
+  }
+}
+
+// network input: asinh(x / sigma) of each 64 x 64 cutout (one sigma per cutout), bf16
+__global__ __launch_bounds__(256) void asinh_input_kernel(const float* x, const float* sigma, int Bn,
+                                                          int npx, uint16_t* out) {
+  const int64_t total = (int64_t)Bn * npx;
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int b = (int)(e / npx);
+    out[e] = to_bf16(asinhf(x[e] / sigma[b]));
+  }
+}
+
+inline dim3 grid_for(int64_t n) { return dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)); }
+
+}  // namespace
+
+int gemm_bf16(int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, const uint16_t* B,
+              int64_t sbk, int64_t sbn, float* C, int64_t ldc, const float* bias, int relu,
+              uint16_t* Cb, int accumulate, int splits, float* ws, int64_t ws_floats,
+              hipStream_t stream) {
+  if (M < 0 || N < 0 || K < 0 || !A || !B || !C || ldc < N || splits < 1)
+    return set_error(APX_E_INVALID, "gemm_bf16: bad arguments");
+  if (M == 0 || N == 0) return APX_OK;
+  const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, splits);
+  if (grid.y > 65535 || splits > 65535) return set_error(APX_E_LIMIT, "gemm_bf16: M or splits too large");
+  // k range per split: a multiple of the k-step
+  const int kchunk = splits == 1 ? std::max(K, 1) : ((K + splits - 1) / splits + BK - 1) / BK * BK;
+  if (splits > 1) {
+    if (!ws || ws_floats < (int64_t)splits * M * N)
+      return set_error(APX_E_INVALID, "gemm_bf16: split-K workspace too small");
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, stream, M, N, K, A, sam, sak, B, sbk, sbn,
+                       C, ldc, bias, relu, Cb, accumulate, ws, kchunk);
+    hipLaunchKernelGGL(splitk_sum_kernel, grid_for((int64_t)M * N), dim3(256), 0, stream, ws, splits, M, N,
+                       C, ldc, bias, relu, Cb, accumulate);
+  } else {
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, stream, M, N, K, A, sam, sak, B, sbk, sbn,
+                       C, ldc, bias, relu, Cb, accumulate, (float*)nullptr, kchunk);
+  }
+  return check_launch("gemm_bf16");
+}
+
+int mse(const float* y, const float* t, const float* w, int Bn, int N, float* dy, float* loss,
+        hipStream_t stream) {
+  if (!y || !t || !w || !dy || !loss || Bn < 1 || N < 1) return set_error(APX_E_INVALID, "mse: bad arguments");
+  hipLaunchKernelGGL(mse_kernel, dim3(1), dim3(256), 0, stream, y, t, w, Bn, N, dy, loss);
+  return check_launch("mse");
+}
+
+int im2col_s2(const uint16_t* X, int Bn, int H, int W, int Cin, int Kp, uint16_t* col,
+              hipStream_t stream) {
+  if (!X || !col || Bn < 1 || H < 1 || W < 1 || Cin < 1 || Kp < 9 * Cin)
+    return set_error(APX_E_INVALID, "im2col: bad arguments");
+  const int64_t n = (int64_t)Bn * ((H + 1) / 2) * ((W + 1) / 2) * Kp;
+  hipLaunchKernelGGL(im2col_kernel, grid_for(n), dim3(256), 0, stream, X, Bn, H, W, Cin, Kp, col);
+  return check_launch("im2col");
+}
+
+int col2im_s2(const float* dcol, int Bn, int H, int W, int Cin, int Kp, float* dX, hipStream_t stream) {
+  if (!dcol || !dX || Bn < 1 || H < 1 || W < 1 || Cin < 1 || Kp < 9 * Cin)
+    return set_error(APX_E_INVALID, "col2im: bad arguments");
+  const int64_t n = (int64_t)Bn * H * W * Cin;
+  hipLaunchKernelGGL(col2im_kernel, grid_for(n), dim3(256), 0, stream, dcol, Bn, H, W, Cin, Kp, dX);
+  return check_launch("col2im");
+}
+
+int cast_gate(const float* src, const float* mask, int64_t n, uint16_t* dst, float* dstf,
+              hipStream_t stream) {
+  if (!src || !dst || n < 0) return set_error(APX_E_INVALID, "cast_gate: bad arguments");
+  if (n == 0) return APX_OK;
+  hipLaunchKernelGGL(cast_gate_kernel, grid_for(n), dim3(256), 0, stream, src, mask, n, dst, dstf);
+  return check_launch("cast_gate");
+}
+
+int colsum(const float* X, int M, int N, float* out, hipStream_t stream) {
+  if (!X || !out || M < 0 || N < 1) return set_error(APX_E_INVALID, "colsum: bad arguments");
+  hipLaunchKernelGGL(colsum_kernel, dim3((N + 63) / 64), dim3(256), 0, stream, X, M, N, out);
+  return check_launch("colsum");
+}
+
+int adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+         float eps, int step, uint16_t* wb, hipStream_t stream) {
+  if (!w || !g || !m || !v || n < 0 || step < 1) return set_error(APX_E_INVALID, "adam: bad arguments");
+  if (n == 0) return APX_OK;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, grid_for(n), dim3(256), 0, stream, w, g, m, v, n, lr, b1, b2, eps, c1,
+                     c2, wb);
+  return check_launch("adam");
+}
+
+int asinh_input(const float* x, const float* sigma, int Bn, int npx, uint16_t* out, hipStream_t stream) {
+  if (!x || !sigma || !out || Bn < 1 || npx < 1) return set_error(APX_E_INVALID, "asinh_input: bad arguments");
+  hipLaunchKernelGGL(asinh_input_kernel, grid_for((int64_t)Bn * npx), dim3(256), 0, stream, x, sigma, Bn,
+                     npx, out);
+  return check_launch("asinh_input");
+}
+
+}  // namespace apx

@@ -1,0 +1,336 @@
+// Moment-based initial Sersic parameters for every cutout of a batch: the
+// device counterpart of the reference's initialise chain, which runs on the
+// host with numpy / scipy:
+//   centre  iterative light-weighted centre of mass in a window
+//           (utils/initialize/center.py:8-52, models/model_object.py:176-233)
+//   PA      angular centre of mass of the edge-subtracted light
+//           (utils/angle_operations.py:35-56, galaxy_model_object.py:78-96)
+//   q       here: PSF-corrected second moments of the pixels above 3 sigma of
+//           the edge (the reference scans isophote amplitudes over
+//           q in [0.2, 0.9], galaxy_model_object.py:97-113)
+//   Re, n   here: the half-light radius and the R80 / R20 concentration of
+//           the elliptical growth curve, inverted through the Sersic growth
+//           curve (the reference fits the binned radial profile with
+//           Nelder-Mead, _shared_methods.py:71-164)
+//   Ie      from the total flux, Re, n and q (utils/conversions/functions.py:168-190)
+// One workgroup per cutout.  The edge level is the mean of the border pixels
+// and its scatter their standard deviation (the reference takes the median
+// and half the 16-84 inter-percentile range).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "capi_internal.h"
+#include "profile.h"
+
+namespace apx {
+
+namespace {
+
+constexpr int NP = APX_BATCH_NP;
+constexpr int kNB = 64;      // growth-curve bins (log-spaced) after the r < kR0 bin
+constexpr float kR0 = 0.25f;
+constexpr int kNT = 64;      // concentration table size
+
+struct ConcTable {
+  float n[kNT];  // Sersic index, increasing
+  float c[kNT];  // R80 / R20 of that index, increasing
+};
+
+__device__ __forceinline__ float block_sum(float v, float* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__device__ __forceinline__ double block_sumd(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void moment_init_kernel(apx_batch b, ConcTable T, double* rep) {
+  __shared__ float redf[4];
+  __shared__ double redd[4];
+  __shared__ float hist[kNB + 1];
+  __shared__ float cen[2];
+  __shared__ int stop;
+  for (int g = blockIdx.x; g < b.ngal; g += gridDim.x) {
+    const apx_batch_galaxy G = b.gal[g];
+    const float* im = b.data + G.pix_off;
+    const int nx = G.nx, ny = G.ny, n = nx * ny;
+    // ---- edge level and scatter
+    const int ne = nx > 1 && ny > 1 ? 2 * nx + 2 * (ny - 2) : n;
+    double s1 = 0.0, s2 = 0.0;
+    for (int e = threadIdx.x; e < ne; e += 256) {
+      int i, j;
+      if (ne == n) {
+        i = e % nx, j = e / nx;
+      } else if (e < nx) {
+        i = e, j = 0;
+      } else if (e < 2 * nx) {
+        i = e - nx, j = ny - 1;
+      } else {
+        const int k = e - 2 * nx;
+        i = (k & 1) ? nx - 1 : 0, j = 1 + (k >> 1);
+      }
+      const double v = im[(int64_t)j * nx + i];
+      s1 += v;
+      s2 += v * v;
+    }
+    s1 = block_sumd(s1, redd);
+    s2 = block_sumd(s2, redd);
+    const float edge = (float)(s1 / ne);
+    const float scatter = (float)sqrt(fmax(s2 / ne - (s1 / ne) * (s1 / ne), 0.0));
+    // ---- PSF second moment (per axis), for the moment corrections
+    const float* K = b.psf + G.psf_off;
+    double k0 = 0.0, k2 = 0.0;
+    for (int e = threadIdx.x; e < G.ph * G.pw; e += 256) {
+      const double dx = e % G.pw - (G.pw - 1) / 2, dy = e / G.pw - (G.ph - 1) / 2;
+      k0 += K[e];
+      k2 += K[e] * (dx * dx + dy * dy);
+    }
+    k0 = block_sumd(k0, redd);
+    k2 = block_sumd(k2, redd);
+    const float psf2 = (float)(0.5 * k2 / fmax(k0, 1e-30));  // sigma^2 per axis
+    // ---- centre: windowed centre of mass from the cutout centre (row, col)
+    int win = max(min(min(nx, ny) / 10, 30), 6);
+    win += win % 2;
+    if (threadIdx.x == 0) {
+      cen[0] = 0.5f * (ny - 1);  // row
+      cen[1] = 0.5f * (nx - 1);  // col
+      stop = 0;
+    }
+    __syncthreads();
+    const float init_r = cen[0], init_c = cen[1];
+    for (int it = 0; it < 100; ++it) {
+      const int r0 = (int)rintf(cen[0]) - win / 2, c0 = (int)rintf(cen[1]) - win / 2;
+      if (r0 < 0 || c0 < 0 || r0 + win >= ny || c0 + win >= nx) {  // image edge: keep the start
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          cen[0] = init_r;
+          cen[1] = init_c;
+        }
+        __syncthreads();
+        break;
+      }
+      float d = 0.0f, sr = 0.0f, sc = 0.0f;
+      for (int e = threadIdx.x; e < win * win; e += 256) {
+        const int rr = e / win, cc = e % win;
+        const float v = im[(int64_t)(r0 + rr) * nx + c0 + cc];
+        d += v;
+        sr += v * rr;
+        sc += v * cc;
+      }
+      d = block_sum(d, redf);
+      sr = block_sum(sr, redf);
+      sc = block_sum(sc, redf);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const float nr = r0 + sr / d, nc = c0 + sc / d;
+        // converged: keep the previous centre, as center.py:45-49 does
+        if (!(fabsf(nr - cen[0]) + fabsf(nc - cen[1]) >= 0.1f)) {
+          stop = 1;
+          if (!isfinite(nr) || !isfinite(nc)) {
+            cen[0] = init_r;
+            cen[1] = init_c;
+          }
+        } else {
+          cen[0] = nr;
+          cen[1] = nc;
+        }
+      }
+      __syncthreads();
+      if (stop) break;
+    }
+    const float pr = cen[0], pc = cen[1];  // pixel indices of the centre
+    // ---- PA: angular centre of mass of (data - edge) about the centre
+    float cs = 0.0f, sn = 0.0f;
+    for (int p = threadIdx.x; p < n; p += 256) {
+      const float X = (p % nx) - pc, Y = (p / nx) - pr;
+      const float w = im[p] - edge;
+      const float r2 = X * X + Y * Y;
+      if (r2 > 0.0f) {
+        const float inv = 1.0f / r2;
+        cs += w * (X * X - Y * Y) * inv;  // cos 2 theta
+        sn += w * (2.0f * X * Y) * inv;   // sin 2 theta
+      }
+    }
+    cs = block_sum(cs, redf);
+    sn = block_sum(sn, redf);
+    float ang = 0.5f * atan2f(sn, cs);  // angle of the light, mod pi
+    // ---- q: second moments along / across that angle, pixels above 3 sigma
+    const float ca = cosf(ang), sa = sinf(ang);
+    const float thr = 3.0f * scatter;
+    float mu = 0.0f, mv = 0.0f, mw = 0.0f;
+    for (int p = threadIdx.x; p < n; p += 256) {
+      const float w = im[p] - edge;
+      if (!(w > thr)) continue;
+      const float X = (p % nx) - pc, Y = (p / nx) - pr;
+      const float u = ca * X + sa * Y, v = -sa * X + ca * Y;
+      mu += w * u * u;
+      mv += w * v * v;
+      mw += w;
+    }
+    mu = block_sum(mu, redf);
+    mv = block_sum(mv, redf);
+    mw = block_sum(mw, redf);
+    float q = 0.7f;
+    if (mw > 0.0f) {
+      const float a2 = fmaxf(mu / mw - psf2, 0.25f), b2 = fmaxf(mv / mw - psf2, 0.25f * 0.04f);
+      q = sqrtf(b2 / a2);
+    }
+    if (q > 1.0f) {  // the light is wider across the angle: swap the axes
+      q = 1.0f / q;
+      ang += 0.5f * (float)kPi;
+    }
+    q = fminf(fmaxf(q, 0.1f), 0.95f);
+    const float cu = cosf(ang), su = sinf(ang);
+    // ---- growth curve in elliptical radius
+    const float rmax = 0.5f * sqrtf((float)(nx * nx + ny * ny)) + 1.0f;
+    const float lstep = logf(rmax / kR0) / kNB;
+    for (int k = threadIdx.x; k <= kNB; k += 256) hist[k] = 0.0f;
+    __syncthreads();
+    for (int p = threadIdx.x; p < n; p += 256) {
+      const float X = (p % nx) - pc, Y = (p / nx) - pr;
+      const float u = cu * X + su * Y, v = (-su * X + cu * Y) / q;
+      const float R = sqrtf(u * u + v * v);
+      const int k = R < kR0 ? 0 : min(kNB, 1 + (int)(logf(R / kR0) / lstep));
+      atomicAdd(&hist[k], im[p] - edge);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float tot = 0.0f;
+      for (int k = 0; k <= kNB; ++k) tot += hist[k];
+      // radius (upper bin edge, linear within a bin) where the curve first reaches f * tot
+      auto radius = [&](float f) {
+        float cum = 0.0f, prev_r = 0.0f;
+        for (int k = 0; k <= kNB; ++k) {
+          const float r_hi = kR0 * expf(lstep * k);
+          const float nc = cum + hist[k];
+          if (nc >= f * tot && hist[k] > 0.0f) return prev_r + (r_hi - prev_r) * (f * tot - cum) / hist[k];
+          cum = nc;
+          prev_r = r_hi;
+        }
+        return rmax;
+      };
+      float r20 = radius(0.2f), r50 = radius(0.5f), r80 = radius(0.8f);
+      // remove the PSF in quadrature (half-light radius of a Gaussian: 1.1774 sigma)
+      const float pw2 = 1.1774f * 1.1774f * psf2;
+      const float re = sqrtf(fmaxf(r50 * r50 - pw2, 0.25f));
+      const float cc = r80 / fmaxf(r20, 1e-3f);
+      float ns = T.n[0];
+      if (cc >= T.c[kNT - 1]) {
+        ns = T.n[kNT - 1];
+      } else if (cc > T.c[0]) {
+        int k = 0;
+        while (k < kNT - 2 && T.c[k + 1] < cc) ++k;
+        const float t = (cc - T.c[k]) / (T.c[k + 1] - T.c[k]);
+        ns = T.n[k] + t * (T.n[k + 1] - T.n[k]);
+      }
+      ns = fminf(fmaxf(ns, 0.4f), 7.5f);
+      // Ie from the total flux (functions.py:168-190)
+      const double bn = sersic_b((double)ns);
+      const double flux_per_ie = 2.0 * kPi * (double)re * re * q * ns * exp(bn) * pow(bn, -2.0 * ns) *
+                                 exp(lgamma(2.0 * ns));
+      const double ie = log10(fmax((double)tot, 1e-30) / flux_per_ie);
+      // plane coordinates (pixel i at origin + i + 0.5 for the batch's pixel matrix)
+      const double px = (double)pc + 0.5, py = (double)pr + 0.5;
+      const double cx = G.origin[0] + b.P[0] * px + b.P[1] * py;
+      const double cy = G.origin[1] + b.P[2] * px + b.P[3] * py;
+      // PA parameter: light angle + north, mod pi (galaxy_model_object.py:94)
+      double pa = fmod((double)ang + 0.5 * kPi, kPi);
+      if (pa < 0.0) pa += kPi;
+      double* r = rep + (int64_t)g * NP;
+      // value -> representation (utils/conversions/optimization.py:6-14)
+      r[0] = cx;
+      r[1] = cy;
+      r[2] = tan((double)q * kPi - 0.5 * kPi);
+      r[3] = pa;
+      r[4] = tan(((double)ns - 0.36) * kPi / (8.0 - 0.36) - 0.5 * kPi);
+      r[5] = (double)re - 1.0 / (double)re;
+      r[6] = ie;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- host: Sersic growth curve R80 / R20 as a function of n ------------
+// regularised lower incomplete gamma P(a, x): series for x < a + 1, else the
+// continued fraction of Q (Legendre's), both to 1e-12
+double gamma_p(double a, double x) {
+  if (x <= 0.0) return 0.0;
+  const double lg = lgamma(a);
+  if (x < a + 1.0) {
+    double term = 1.0 / a, sum = term;
+    for (int k = 1; k < 1000; ++k) {
+      term *= x / (a + k);
+      sum += term;
+      if (fabs(term) < fabs(sum) * 1e-13) break;
+    }
+    return sum * exp(-x + a * log(x) - lg);
+  }
+  double bb = x + 1.0 - a, c = 1e300, d = 1.0 / bb, h = d;
+  for (int k = 1; k < 1000; ++k) {
+    const double an = -k * (k - a);
+    bb += 2.0;
+    d = an * d + bb;
+    if (fabs(d) < 1e-300) d = 1e-300;
+    c = bb + an / c;
+    if (fabs(c) < 1e-300) c = 1e-300;
+    d = 1.0 / d;
+    const double del = d * c;
+    h *= del;
+    if (fabs(del - 1.0) < 1e-13) break;
+  }
+  return 1.0 - exp(-x + a * log(x) - lg) * h;
+}
+
+// x with P(a, x) = f, by bisection on a bracketing interval
+double gamma_p_inv(double a, double f) {
+  double lo = 0.0, hi = a + 10.0 * sqrt(a) + 10.0;
+  while (gamma_p(a, hi) < f) hi *= 2.0;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (gamma_p(a, mid) < f) lo = mid;
+    else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+const ConcTable& conc_table() {
+  static ConcTable T = [] {
+    ConcTable t;
+    for (int k = 0; k < kNT; ++k) {
+      const double n = 0.36 * pow(8.0 / 0.36, (double)k / (kNT - 1));
+      // L(<R) / L = P(2n, b_n (R / Re)^(1/n))  =>  R80 / R20 = (x80 / x20)^n
+      const double x20 = gamma_p_inv(2.0 * n, 0.2), x80 = gamma_p_inv(2.0 * n, 0.8);
+      t.n[k] = (float)n;
+      t.c[k] = (float)pow(x80 / x20, n);
+    }
+    return t;
+  }();
+  return T;
+}
+
+}  // namespace
+
+double sersic_light_fraction(double n, double r_over_re) {
+  return gamma_p(2.0 * n, sersic_b(n) * pow(r_over_re, 1.0 / n));
+}
+
+int batch_moment_init(const apx_batch* b, double* rep, hipStream_t stream) {
+  if (!b || !b->gal || !b->data || !b->psf || !rep || b->ngal < 1)
+    return set_error(APX_E_INVALID, "batch_moment_init: bad arguments");
+  hipLaunchKernelGGL(moment_init_kernel, dim3(std::min(b->ngal, 4096)), dim3(256), 0, stream, *b,
+                     conc_table(), rep);
+  return check_launch("batch_moment_init");
+}
+
+}  // namespace apx

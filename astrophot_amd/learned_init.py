@@ -1,0 +1,190 @@
+"""Training and use of the learned initialiser (astrophot_amd.cnn).
+
+* Training data: 64 x 64 Sersic cutouts that this package's batched renderer
+  draws. The priors are the batched-fit ones (population.sample_population),
+  with Re held to what fits 64 px. Gaussian noise matches the drawn S/N. A
+  fixed pool of PSFs is re-rendered with fresh parameters and noise every
+  pass.
+* Use: `learned_fit` starts every LM from the network's proposal. Fits that
+  do not converge (status not 1 or 2) are fitted again from the moment-based
+  initialiser, which stays the fallback and the judge.
+
+    python -m astrophot_amd.learned_init --samples 1000000 --out astrophot_amd/data/init_net.npz
+"""
+
+from __future__ import annotations
+
+import argparse
+import math
+import time
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from . import cnn
+from .batch import BatchFit, Cutout, rep_to_values, values_to_rep
+from .population import sample_population
+from .utils import gaussian_psf
+
+SIDE = cnn.SIDE
+RE_RANGE = (1.5, 16.0)      # Re that fits a 64 px cutout
+WEIGHTS_PATH = Path(__file__).resolve().parent / "data" / "init_net.npz"
+CONVERGED = (1, 2)          # batch statuses that count as a converged fit
+
+
+def c4_population(n: int, seed: int, make_psfs: bool = True):
+    return sample_population(n, seed, side_range=(SIDE, SIDE), re_range=RE_RANGE, make_psfs=make_psfs)
+
+
+def render_noisy(bf: BatchFit, truth: np.ndarray, sky_sigma: np.ndarray, gen: torch.Generator):
+    """Truth rendered by the batch renderer plus Gaussian noise; (n, 64, 64) float32 on the device."""
+    n = truth.shape[0]
+    img = bf.render(values_to_rep(truth)).view(n, SIDE, SIDE)
+    sig = torch.as_tensor(sky_sigma, dtype=torch.float32, device=img.device)
+    img += sig[:, None, None] * torch.randn(img.shape, generator=gen, device=img.device)
+    return img, sig
+
+
+class Trainer:
+    """Trains an InitNet on renderer-generated cutouts with Adam (cosine learning rate)."""
+
+    def __init__(self, net: cnn.InitNet, pool: int = 8192, seed: int = 0):
+        self.net, self.pool = net, pool
+        self.rng = np.random.default_rng(seed)
+        sig = self.rng.uniform(1.0, 3.0, pool)
+        psfs = {}
+        cuts = []
+        for s in sig:  # PSF boxes quantised to 0.05 px of sigma to share the host work
+            key = round(float(s) * 20) / 20
+            if key not in psfs:
+                psfs[key] = gaussian_psf(key, 2 * math.ceil(4 * key) + 1, 1.0)
+            cuts.append(Cutout(np.broadcast_to(0.0, (SIDE, SIDE)), None, psfs[key]))
+        dev = net.device
+        zeros = torch.zeros(pool * SIDE * SIDE, dtype=torch.float32, device=dev)
+        self.bf = BatchFit(cuts, data=zeros, variance=torch.ones_like(zeros))
+        self.gen = torch.Generator(device=dev)
+        self.gen.manual_seed(seed + 1)
+        self.weights = torch.as_tensor(cnn.ENC_WEIGHTS, device=dev)
+
+    def draw(self):
+        pop = c4_population(self.pool, int(self.rng.integers(1 << 31)), make_psfs=False)
+        img, sig = render_noisy(self.bf, pop.truth, pop.sky_sigma, self.gen)
+        tgt = torch.as_tensor(cnn.encode(pop.truth, pop.sky_sigma), device=img.device)
+        return img, sig, tgt
+
+    def train(self, steps: int, batch: int = 256, lr: float = 2e-3, warmup: int = 100, log=None) -> np.ndarray:
+        net = self.net
+        per = self.pool // batch
+        hist = torch.zeros(steps, dtype=torch.float32, device=net.device)
+        img = sig = tgt = None
+        t0 = time.perf_counter()
+        for step in range(steps):
+            if step % per == 0:
+                img, sig, tgt = self.draw()
+            sl = slice((step % per) * batch, (step % per + 1) * batch)
+            out = net.forward(net.input(img[sl], sig[sl]), batch)
+            dout, loss = net.loss_grad(out, tgt[sl].contiguous(), self.weights, batch)
+            net.backward(batch, dout)
+            frac = step / max(steps - 1, 1)
+            lr_t = lr * min(1.0, (step + 1) / warmup) * 0.5 * (1 + math.cos(math.pi * frac))
+            net.adam_step(lr_t)
+            hist[step:step + 1].copy_(loss)
+            if log and (step + 1) % log == 0:
+                print(f"step {step + 1}/{steps} loss {float(hist[max(0, step - log + 1):step + 1].mean()):.4f} "
+                      f"({time.perf_counter() - t0:.1f} s)", flush=True)
+        return hist.cpu().numpy()
+
+
+def load_net(path: Optional[Path] = None, device: str = "cuda") -> cnn.InitNet:
+    return cnn.InitNet.load(path or WEIGHTS_PATH, device=device)
+
+
+def cutout_images(bf: BatchFit) -> torch.Tensor:
+    n = len(bf.gal_host)
+    return bf.data.view(n, SIDE, SIDE)
+
+
+def learned_fit(bf: BatchFit, net: cnn.InitNet, sigma: torch.Tensor) -> Dict:
+    """LM of every cutout from the network's proposal, with the moment-init
+    fallback for the fits that do not converge.  Returns the merged results,
+    the per-path results and timings."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v = net.predict(cutout_images(bf), sigma)
+    origin = np.array([[g.origin[0], g.origin[1]] for g in bf.gal_host])
+    v[:, :2] += origin
+    rep_c = torch.as_tensor(values_to_rep(v), dtype=torch.float64, device=bf.data.device)
+    res_c = bf.fit(rep_c)
+    fail = ~np.isin(res_c["status"], CONVERGED)
+    res = {k: (v_.copy() if isinstance(v_, np.ndarray) else v_) for k, v_ in res_c.items()}
+    res_f = None
+    if fail.any():
+        rep_m = bf.moment_init()
+        mask = torch.as_tensor(fail, device=rep_m.device)
+        rep_f = torch.where(mask[:, None], rep_m, torch.full_like(rep_m, float("nan")))
+        res_f = bf.fit(rep_f)
+        for k in ("rep", "values", "chi2", "status"):
+            res[k][fail] = res_f[k][fail]
+        res["iterations"] = res_c["iterations"] + np.where(fail, res_f["iterations"], 0)
+    torch.cuda.synchronize()
+    res["seconds"] = time.perf_counter() - t0
+    res["fallback"] = fail
+    res["cnn_values"] = v
+    res["ticks"] = res_c["ticks"] + (res_f["ticks"] if res_f else 0)
+    return res
+
+
+def moment_fit(bf: BatchFit) -> Dict:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = bf.fit(bf.moment_init())
+    torch.cuda.synchronize()
+    res["seconds"] = time.perf_counter() - t0
+    return res
+
+
+def c4_problem(n: int, seed: int):
+    """n seeded 64 x 64 cutouts with their own PSFs, rendered and noised on the device."""
+    pop = c4_population(n, seed, make_psfs=False)
+    # PSF sigma quantised to 0.01 px (the images are rendered with the quantised PSF)
+    pop.psf_sigma = np.round(pop.psf_sigma, 2)
+    cache = {}
+    for s in np.unique(pop.psf_sigma):
+        cache[float(s)] = gaussian_psf(float(s), 2 * math.ceil(4 * float(s)) + 1, 1.0)
+    pop.psfs = [cache[float(s)] for s in pop.psf_sigma]
+    cuts = [Cutout(np.broadcast_to(0.0, (SIDE, SIDE)), None, p) for p in pop.psfs]
+    dev = "cuda"
+    zeros = torch.zeros(n * SIDE * SIDE, dtype=torch.float32, device=dev)
+    bf0 = BatchFit(cuts, data=zeros, variance=torch.ones_like(zeros))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 1)
+    img, sig = render_noisy(bf0, pop.truth, pop.sky_sigma, gen)
+    data = img.reshape(-1).clone()
+    var = (sig.double() ** 2).repeat_interleave(SIDE * SIDE)
+    del bf0
+    return pop, cuts, data, var, sig
+
+
+def main():
+    ap_ = argparse.ArgumentParser(description="train the learned initialiser on renderer-generated cutouts")
+    ap_.add_argument("--samples", type=int, default=1_000_000)
+    ap_.add_argument("--batch", type=int, default=256)
+    ap_.add_argument("--lr", type=float, default=2e-3)
+    ap_.add_argument("--pool", type=int, default=8192)
+    ap_.add_argument("--seed", type=int, default=0)
+    ap_.add_argument("--out", type=str, default=str(WEIGHTS_PATH))
+    args = ap_.parse_args()
+    net = cnn.InitNet(seed=args.seed)
+    tr = Trainer(net, pool=args.pool, seed=args.seed)
+    steps = args.samples // args.batch
+    hist = tr.train(steps, batch=args.batch, lr=args.lr, log=max(steps // 20, 1))
+    Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    net.save(args.out)
+    np.save(str(Path(args.out).with_suffix("")) + "_loss.npy", hist)
+    print(f"saved {args.out}; loss first 100 {hist[:100].mean():.4f}, last 100 {hist[-100:].mean():.4f}")
+
+
+if __name__ == "__main__":
+    main()

@@ -37,7 +37,7 @@ class Population:
 
 def sample_population(n: int, seed: int, side_range=(32, 256), re_range=(2.0, 40.0), n_range=(0.5, 6.0),
                       q_range=(0.2, 1.0), snr_range=(10.0, 1000.0), sigma_range=(1.0, 3.0),
-                      ie_range=(-0.5, 1.5)) -> Population:
+                      ie_range=(-0.5, 1.5), make_psfs=True) -> Population:
     rng = np.random.default_rng(seed)
     sides = np.rint(np.exp(rng.uniform(math.log(side_range[0]), math.log(side_range[1]), n))).astype(int)
     cx = sides / 2 + rng.uniform(-1, 1, n)
@@ -51,10 +51,9 @@ def sample_population(n: int, seed: int, side_range=(32, 256), re_range=(2.0, 40
     snr = np.exp(rng.uniform(math.log(snr_range[0]), math.log(snr_range[1]), n))
     sig = rng.uniform(*sigma_range, n)
     psfs, sky = [], np.empty(n)
-    cache = {}
     for k in range(n):
-        side = 2 * math.ceil(4 * sig[k]) + 1
-        psfs.append(gaussian_psf(sig[k], side, 1.0))
+        if make_psfs:
+            psfs.append(gaussian_psf(sig[k], 2 * math.ceil(4 * sig[k]) + 1, 1.0))
         F = sersic_total_flux(10 ** ie[k], ns[k], re[k], q[k])
         sky[k] = F / (snr[k] * math.sqrt(math.pi * re[k] ** 2 * q[k]))
     truth = np.stack([cx, cy, q, pa, ns, re, ie], axis=1)

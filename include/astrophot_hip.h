@@ -262,7 +262,24 @@ int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int3
 int apx_batch_jacobian(const apx_batch* batch, const double* rep, float* out, int32_t parts,
                        void* stream);
 
-/* Start the fits from rep0 (device, ngal x 7). */
+/* Moment-based initial parameters of every cutout from its data alone, into
+ * rep (device, ngal x 7, representation space): the batched counterpart of
+ * the initialise chain Component_Model.initialize (models/model_object.py:
+ * 176-233, utils/initialize/center.py:8-52), Galaxy_Model.initialize
+ * (models/galaxy_model_object.py:56-113) and parametric_initialize
+ * (models/_shared_methods.py:71-164).  Centre and PA follow the reference's
+ * estimators; q, Re, n and Ie come from PSF-corrected moments and the
+ * elliptical growth curve instead of the reference's isophote scan and
+ * Nelder-Mead profile fit. */
+int apx_batch_moment_init(const apx_batch* batch, double* rep, void* stream);
+
+/* Fraction of a Sersic profile's light inside elliptical radius r (in units
+ * of Re): P(2n, b_n r^(1/n)), the growth curve the initialiser inverts
+ * (host function; utils/conversions/functions.py:7-22 for b_n). */
+double apx_sersic_light_fraction(double n, double r_over_re);
+
+/* Start the fits from rep0 (device, ngal x 7).  A fit whose row holds a
+ * non-finite value is not run: it ends at once with status 5 (skipped). */
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream);
 
 /* One LM tick of every unfinished fit. */
@@ -270,13 +287,63 @@ int apx_batch_tick(const apx_batch* batch, void* stream);
 
 /* Results (device arrays): best state (res(), fit/base.py:104-117), its chi2
  * (reduced), status (0 running, 1 success, 2 success by immobility, 3 fail:
- * no improving step, 4 fail: max iterations), iterations, and the number of
- * finished fits in done[0]. */
+ * no improving step, 4 fail: max iterations, 5 skipped), iterations, and the
+ * number of finished fits in done[0]. */
 int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
                       int32_t* iterations, int32_t* done, void* stream);
 
 /* Last error message of the calling thread. */
 const char* apx_last_error(void);
+
+/* -------------------------------------------------------------------------
+ * Learned initialiser: primitives of a small CNN (4 convolutions 3x3 /
+ * stride 2 / pad 1, 32-32-64-64 channels, then a 2-layer MLP) that maps a
+ * 64 x 64 asinh-normalised cutout to the 7 Sersic parameters LM starts from.
+ * The reference has no counterpart: its LM starts from the initialise chain
+ * (apx_batch_moment_init above), which stays the fallback.  Bf16 values are
+ * uint16 (round to nearest even); activations are NHWC.
+ * ------------------------------------------------------------------------- */
+
+/* C[M,N] (= or +=) A[M,K] B[K,N] (+ bias[N], ReLU) with BF16 MFMA, float32
+ * accumulate.  A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], C row
+ * pitch ldc; Cb (optional) receives a bf16 copy of C.  splits > 1 divides K
+ * over that many workgroup layers whose partials (in ws, >= splits*M*N
+ * floats) are summed in a fixed order. */
+int apx_gemm_bf16(int32_t M, int32_t N, int32_t K, const uint16_t* A, int64_t sam, int64_t sak,
+                  const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int64_t ldc,
+                  const float* bias, int32_t relu, uint16_t* Cb, int32_t accumulate, int32_t splits,
+                  float* ws, int64_t ws_floats, void* stream);
+
+/* Weighted mean-squared error of B x N outputs y against targets t with
+ * per-column weights w: loss[0] = sum w_j (y - t)^2 / B, dy its gradient. */
+int apx_mse(const float* y, const float* t, const float* w, int32_t B, int32_t N, float* dy,
+            float* loss, void* stream);
+
+/* im2col of a 3x3 / stride-2 / pad-1 convolution of X (B,H,W,C) into
+ * col (B*ceil(H/2)*ceil(W/2), Kp), columns (ky*3+kx)*C + c, zero beyond 9C. */
+int apx_im2col_s2(const uint16_t* X, int32_t B, int32_t H, int32_t W, int32_t C, int32_t Kp,
+                  uint16_t* col, void* stream);
+
+/* The adjoint of apx_im2col_s2: dX (B,H,W,C) float32 from dcol. */
+int apx_col2im_s2(const float* dcol, int32_t B, int32_t H, int32_t W, int32_t C, int32_t Kp,
+                  float* dX, void* stream);
+
+/* dst = bf16(src), zeroed where mask <= 0 when mask is given (ReLU gate);
+ * dstf (optional) receives the gated float32 values. */
+int apx_cast_gate_bf16(const float* src, const float* mask, int64_t n, uint16_t* dst, float* dstf,
+                       void* stream);
+
+/* out[n] = sum_m X[m,n] (bias gradients). */
+int apx_colsum(const float* X, int32_t M, int32_t N, float* out, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1) on float32 weights w with
+ * gradient g and moments m, v; w_bf16 (optional) receives bf16(w). */
+int apx_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
+             float beta2, float eps, int32_t step, uint16_t* w_bf16, void* stream);
+
+/* Network input: out = bf16(asinh(x / sigma[b])) for B cutouts of npx pixels. */
+int apx_asinh_input(const float* x, const float* sigma, int32_t B, int32_t npx, uint16_t* out,
+                    void* stream);
 
 #ifdef __cplusplus
 }

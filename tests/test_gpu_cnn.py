@@ -1,0 +1,122 @@
+"""The learned initialiser's kernels (csrc/cnn.hip) against torch float32 on
+the CPU.
+
+* apx_gemm_bf16: every operand orientation, split-K and the bias / ReLU
+  epilogue, against a float32 matmul of the same bf16-rounded operands
+  (tolerance: float32 accumulation order, 1e-5 of the scale);
+* the network forward pass and every weight and bias gradient against torch
+  autograd on the CPU through the same layers, with the same bf16 rounding
+  points emulated (activations and back-propagated gradients rounded to bf16
+  where the kernels store them);
+* a short training run on renderer-generated cutouts lowers the loss.
+"""
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from astrophot_amd import cnn
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def bf(x):
+    return x.to(torch.bfloat16).to(torch.float32)
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb,split", [(64, 64, 32, False, False, False), (100, 37, 75, False, False, False),
+                                               (33, 130, 300, True, False, False), (70, 20, 96, False, True, False),
+                                               (32, 32, 20000, True, False, True), (288, 32, 9000, True, False, True)])
+def test_gemm_bf16_matches_float32(M, N, K, ta, tb, split):
+    g = torch.Generator().manual_seed(M * 7 + N)
+    A = torch.randn((K, M) if ta else (M, K), generator=g)
+    B = torch.randn((N, K) if tb else (K, N), generator=g)
+    bias = torch.randn(N, generator=g)
+    Ab, Bb = A.to(torch.bfloat16).to(DEV), B.to(torch.bfloat16).to(DEV)
+    sam, sak = (1, M) if ta else (K, 1)
+    sbk, sbn = (1, K) if tb else (N, 1)
+    C = torch.zeros((M, N), device=DEV)
+    Cb = torch.zeros((M, N), dtype=torch.bfloat16, device=DEV)
+    ws = torch.zeros(1 << 22, device=DEV) if split else None
+    cnn.gemm(Ab, sam, sak, Bb, sbk, sbn, C, M, N, K, bias=bias.to(DEV), relu=True, Cb=Cb, ws=ws)
+    Af, Bf = bf(A), bf(B)
+    ref = torch.relu((Af.t() if ta else Af).double() @ (Bf.t() if tb else Bf).double() + bias.double())
+    scale = float((Af.abs().double().t() if ta else Af.abs().double()).max() * (Bf.abs().max()) * K ** 0.5)
+    assert float((C.cpu().double() - ref).abs().max()) < 1e-5 * scale
+    assert torch.equal(Cb.cpu(), C.cpu().to(torch.bfloat16))
+
+
+def torch_reference(net, x, dout_seed=3):
+    """The same network in torch on the CPU (float32), bf16 rounding where the kernels round."""
+    params = []
+    for lay in net.layers:
+        w = bf(lay.w.cpu()).requires_grad_(True)
+        b = lay.b.cpu().clone().requires_grad_(True)
+        params.append((w, b))
+    h = bf(x.cpu().float())
+    acts = []
+    for (cin, cout), (w, b) in zip(cnn.CONV, params[:4]):
+        # w rows (ky, kx, ci) -> conv weight (cout, cin, 3, 3)
+        W = w[:9 * cin].reshape(3, 3, cin, cout).permute(3, 2, 0, 1)
+        z = F.conv2d(h, W, b, stride=2, padding=1)
+        y = torch.relu(z)
+        acts.append(y)
+        h = RoundBF.apply(y)
+    B = x.shape[0]
+    flat = h.permute(0, 2, 3, 1).reshape(B, -1)  # NHWC flatten
+    (w5, b5), (w6, b6) = params[4:]
+    hid = torch.relu(flat @ w5 + b5)
+    out = RoundBF.apply(hid) @ w6 + b6
+    return out, params
+
+
+class RoundBF(torch.autograd.Function):
+    """bf16 rounding in the forward pass; the backward rounds the incoming
+    gradient to bf16, as the kernels do before every gradient GEMM."""
+
+    @staticmethod
+    def forward(ctx, x):
+        return bf(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return bf(g)
+
+
+def test_network_forward_and_gradients_match_torch():
+    torch.manual_seed(0)
+    net = cnn.InitNet(seed=1)
+    B = 8
+    img = torch.randn(B, 64, 64) * 3.0
+    sigma = torch.full((B,), 1.5)
+    x = net.input(img.to(DEV), sigma.to(DEV))
+    out = net.forward(x, B).clone()
+    xr = torch.asinh(img / 1.5).reshape(B, 1, 64, 64)
+    ref, params = torch_reference(net, xr)
+    scale = float(ref.abs().max())
+    assert float((out.cpu() - ref.detach()).abs().max()) < 2e-2 * scale
+    g = torch.Generator().manual_seed(5)
+    dout = torch.randn((B, cnn.NOUT), generator=g)
+    # the GPU backward takes dout rounded to bf16; give the CPU the same
+    dout = bf(dout)
+    net.backward(B, dout.to(DEV).contiguous())
+    ref.backward(dout)
+    for i, (lay, (w, b)) in enumerate(zip(net.layers, params)):
+        gw_ref = w.grad
+        gw = lay.gw.cpu()
+        s = float(gw_ref.abs().max()) + 1e-12
+        assert float((gw - gw_ref).abs().max()) < 3e-2 * s, (i, float((gw - gw_ref).abs().max()) / s)
+        s = float(b.grad.abs().max()) + 1e-12
+        assert float((lay.gb.cpu() - b.grad).abs().max()) < 3e-2 * s, i
+
+
+def test_short_training_lowers_the_loss():
+    from astrophot_amd.learned_init import Trainer
+    net = cnn.InitNet(seed=2)
+    tr = Trainer(net, pool=2048, seed=4)
+    losses = tr.train(steps=240, batch=128, lr=2e-3)
+    first, last = np.mean(losses[:20]), np.mean(losses[-20:])
+    assert last < 0.5 * first, (first, last)

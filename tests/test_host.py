@@ -60,6 +60,22 @@ def test_library_workspace_queries_without_gpu():
     assert lib.apx_render_workspace_bytes(1000) >= 8 * 1000
 
 
+def test_sersic_light_fraction_matches_scipy():
+    """The host growth curve the moment initialiser inverts, P(2n, b_n r^(1/n)),
+    against scipy's regularised incomplete gamma; and the half-light property
+    that defines b_n (exact up to the b_n polynomial's own accuracy)."""
+    from scipy.special import gammainc
+    lib = ctypes.CDLL(str(L.LIB_PATH))
+    f = lib.apx_sersic_light_fraction
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double, ctypes.c_double]
+    for n in (0.36, 0.5, 1.0, 2.5, 4.0, 8.0):
+        b = H.sersic_n_to_b(torch.tensor(n, dtype=torch.float64)).item()
+        for r in (0.05, 0.3, 1.0, 2.0, 7.5):
+            assert abs(f(n, r) - gammainc(2 * n, b * r ** (1 / n))) < 1e-10, (n, r)
+        assert abs(f(n, 1.0) - 0.5) < 2e-3, n
+
+
 def test_product_fails_loudly_without_gpu():
     if torch.cuda.is_available():
         pytest.skip("GPU visible")

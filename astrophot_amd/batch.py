@@ -68,7 +68,9 @@ SIGNATURES = {
     "apx_sersic_light_fraction": (ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
     "apx_batch_start": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
     "apx_batch_tick": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p]),
-    "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 6),
+    "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 7),
+    "apx_batch_start_fallback": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
 }
 L.SIGNATURES.update(SIGNATURES)
 
@@ -166,6 +168,7 @@ class BatchFit:
         self._chi2 = torch.zeros(ng, dtype=torch.float64, device=dev)
         self._status = torch.zeros(ng, dtype=torch.int32, device=dev)
         self._iters = torch.zeros(ng, dtype=torch.int32, device=dev)
+        self._fbran = torch.zeros(ng, dtype=torch.int32, device=dev)
         self._done = torch.zeros(1, dtype=torch.int32, device=dev)
         self.graph = None
         self.ticks = 0
@@ -217,10 +220,18 @@ class BatchFit:
         return out
 
     # -- fitting ------------------------------------------------------------
-    def start(self, rep0):
+    def start(self, rep0, fallback=None):
+        """Start every fit from ``rep0``; with ``fallback`` (ngal x 7) a failed
+        run restarts once from its fallback row within the same ticks."""
         rep0 = torch.as_tensor(rep0, dtype=torch.float64, device=self.data.device).contiguous()
         self._rep0 = rep0
-        L.check(self.lib.apx_batch_start(ctypes.byref(self.b), rep0.data_ptr(), L.stream()))
+        if fallback is None:
+            L.check(self.lib.apx_batch_start(ctypes.byref(self.b), rep0.data_ptr(), L.stream()))
+        else:
+            fallback = torch.as_tensor(fallback, dtype=torch.float64, device=self.data.device).contiguous()
+            self._fb = fallback
+            L.check(self.lib.apx_batch_start_fallback(ctypes.byref(self.b), rep0.data_ptr(), fallback.data_ptr(),
+                                                      L.stream()))
         self.ticks = 0
 
     def tick(self):
@@ -229,8 +240,8 @@ class BatchFit:
 
     def done_count(self) -> int:
         L.check(self.lib.apx_batch_results(ctypes.byref(self.b), self._rep.data_ptr(), self._chi2.data_ptr(),
-                                           self._status.data_ptr(), self._iters.data_ptr(), self._done.data_ptr(),
-                                           L.stream()))
+                                           self._status.data_ptr(), self._iters.data_ptr(),
+                                           self._fbran.data_ptr(), self._done.data_ptr(), L.stream()))
         return int(self._done.item())
 
     def capture(self, ticks_per_graph: int = 4):
@@ -246,10 +257,11 @@ class BatchFit:
         torch.cuda.current_stream().wait_stream(stream)
         self.ticks_per_graph = ticks_per_graph
 
-    def fit(self, rep0, max_ticks: int = 4000, check_every: int = 8, use_graph: bool = True):
-        """Run every fit to completion.  Returns a dict of host arrays: rep,
-        values, chi2 (reduced), status, iterations."""
-        self.start(rep0)
+    def fit(self, rep0, max_ticks: int = 4000, check_every: int = 8, use_graph: bool = True, fallback=None):
+        """Run every fit to completion (a failed run restarts once from its
+        ``fallback`` row when given).  Returns a dict of host arrays: rep,
+        values, chi2 (reduced), status, iterations (all runs), fallback."""
+        self.start(rep0, fallback)
         self.tick()
         if use_graph:
             if self.graph is None:
@@ -269,4 +281,4 @@ class BatchFit:
         rep = self._rep.cpu().numpy()
         return {"rep": rep, "values": rep_to_values(rep), "chi2": self._chi2.cpu().numpy(),
                 "status": self._status.cpu().numpy(), "iterations": self._iters.cpu().numpy(),
-                "ticks": self.ticks}
+                "fallback": self._fbran.cpu().numpy().astype(bool), "ticks": self.ticks}

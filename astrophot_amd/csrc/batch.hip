@@ -50,7 +50,9 @@ struct GalState {
   double scary_dx[8], scary_chi2, scary_L;
   double H[NP * NP], g[NP], h[8], a[8], ha[8], rpp[8];
   double chi2_m0, chi2, ndf;
+  double fb[8];  // fallback start (representation), armed when fb_state == 1
   int32_t nhist, iter, scary_valid, nostep, direction, sub, phase, status;
+  int32_t fb_state, iter_total;  // fallback: 0 none, 1 armed, 2 ran; iterations of earlier runs
 };
 
 struct GalEval {
@@ -938,22 +940,57 @@ __device__ inline void push_hist(GalState* st, double c) {
   }
 }
 
-__global__ void start_kernel(apx_batch b, const double* rep0) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= b.ngal) return;
-  GalState* st = st_of(b, g);
-  const apx_batch_galaxy G = b.gal[g];
-  for (int i = 0; i < 8; ++i) st->x[i] = st->ev[i] = i < NP ? rep0[(int64_t)g * NP + i] : 0.0;
+// a fresh LM run from x0 (lm.py:155-234 state); a non-finite start is not run
+__device__ void begin_run(const apx_batch& b, GalState* st, const double* x0) {
+  bool finite = true;
+  for (int i = 0; i < 8; ++i) {
+    const double v = i < NP ? x0[i] : 0.0;
+    st->x[i] = st->ev[i] = v;
+    finite = finite && isfinite(v);
+  }
   st->L = b.L0;
   st->nhist = 0;
   st->iter = 0;
   st->res_chi2 = INFINITY;
   for (int i = 0; i < NP; ++i) st->res_x[i] = st->x[i];
-  bool finite = true;
-  for (int i = 0; i < NP; ++i) finite = finite && isfinite(st->x[i]);
-  st->phase = finite ? PH_INIT : PH_DONE;  // a non-finite start is not run
+  st->phase = finite ? PH_INIT : PH_DONE;
   st->status = finite ? S_RUN : S_SKIPPED;
+}
+
+__global__ void start_kernel(apx_batch b, const double* rep0, const double* fallback) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= b.ngal) return;
+  GalState* st = st_of(b, g);
+  const apx_batch_galaxy G = b.gal[g];
   st->ndf = fmax(1.0, (double)G.nx * G.ny - NP);
+  st->iter_total = 0;
+  st->fb_state = 0;
+  if (fallback) {
+    bool finite = true;
+    for (int i = 0; i < 8; ++i) {
+      st->fb[i] = i < NP ? fallback[(int64_t)g * NP + i] : 0.0;
+      finite = finite && isfinite(st->fb[i]);
+    }
+    st->fb_state = finite ? 1 : 0;
+  }
+  double x0[NP];
+  for (int i = 0; i < NP; ++i) x0[i] = rep0[(int64_t)g * NP + i];
+  begin_run(b, st, x0);
+}
+
+// the end of a run: a run that failed (no improving step, or max_iter) with
+// an armed fallback starts again from the fallback, once; else the fit is done
+__device__ void end_run(const apx_batch& b, GalState* st, int32_t status) {
+  if ((status == S_NOSTEP || status == S_MAXITER) && st->fb_state == 1) {
+    st->fb_state = 2;
+    st->iter_total += st->iter;
+    double x0[NP];
+    for (int i = 0; i < NP; ++i) x0[i] = st->fb[i];
+    begin_run(b, st, x0);
+    return;
+  }
+  st->status = status;
+  st->phase = PH_DONE;
 }
 
 __global__ void stage_eval_kernel(apx_batch b) {  // Jacobian stage evaluates at x
@@ -1016,8 +1053,7 @@ __device__ void finish_step(const apx_batch& b, GalState* st) {
   double chi2, L;
   if (st->nostep) {
     if (!st->scary_valid) {  // OptimizeStop (lm.py:348-355, 455-459)
-      st->status = S_NOSTEP;
-      st->phase = PH_DONE;
+      end_run(b, st, S_NOSTEP);
       return;
     }
     dx = st->scary_dx;
@@ -1034,16 +1070,18 @@ __device__ void finish_step(const apx_batch& b, GalState* st) {
   Ldn(st);
   st->iter += 1;
   const int n = st->nhist;
+  int32_t status = S_RUN;
   if (n >= 3 && (hist_back(st, 3) - hist_back(st, 1)) / hist_back(st, 1) < b.relative_tolerance &&
       st->L < 0.1) {
-    st->status = S_SUCCESS;
+    status = S_SUCCESS;
   } else if (n > 10 &&
              (hist_back(st, 10) - hist_back(st, 1)) / hist_back(st, 1) < b.relative_tolerance) {
-    st->status = S_IMMOBILE;
+    status = S_IMMOBILE;
   } else if (st->iter >= b.max_iter) {
-    st->status = S_MAXITER;
+    status = S_MAXITER;
   }
-  st->phase = st->status == S_RUN ? PH_NEEDJ : PH_DONE;
+  if (status == S_RUN) st->phase = PH_NEEDJ;
+  else end_run(b, st, status);
 }
 
 __global__ void lm_decide_kernel(apx_batch b) {  // lm.py:293-346
@@ -1098,14 +1136,15 @@ __global__ void lm_decide_kernel(apx_batch b) {  // lm.py:293-346
 }
 
 __global__ void results_kernel(apx_batch b, double* rep, double* chi2, int32_t* status,
-                               int32_t* iters, int32_t* done) {
+                               int32_t* iters, int32_t* fallback, int32_t* done) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= b.ngal) return;
   const GalState* st = st_of(b, g);
   for (int i = 0; i < NP; ++i) rep[(int64_t)g * NP + i] = st->nhist > 0 ? st->res_x[i] : st->x[i];
   if (chi2) chi2[g] = st->res_chi2;
   if (status) status[g] = st->status;
-  if (iters) iters[g] = st->iter;
+  if (iters) iters[g] = st->iter_total + st->iter;
+  if (fallback) fallback[g] = st->fb_state == 2;
   if (done && st->phase == PH_DONE) atomicAdd(done, 1);
 }
 
@@ -1310,11 +1349,11 @@ int batch_jacobian(const apx_batch* b, const double* rep, float* out, int32_t pa
   return run_stage(*b, s, ST_ALL_JAC, EPI_NORMAL, rep, out, stream, parts);
 }
 
-int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream) {
+int batch_start(const apx_batch* b, const double* rep0, const double* fallback, hipStream_t stream) {
   int rc = check_batch(b);
   if (rc) return rc;
   if (!rep0) return set_error(APX_E_INVALID, "batch_start: null pointer");
-  hipLaunchKernelGGL(start_kernel, dim3((b->ngal + 255) / 256), dim3(256), 0, stream, *b, rep0);
+  hipLaunchKernelGGL(start_kernel, dim3((b->ngal + 255) / 256), dim3(256), 0, stream, *b, rep0, fallback);
   return check_launch("batch_start");
 }
 
@@ -1337,11 +1376,11 @@ int batch_tick(const apx_batch* b, hipStream_t stream) {
 }
 
 int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,
-                  int32_t* done, hipStream_t stream) {
+                  int32_t* fallback, int32_t* done, hipStream_t stream) {
   if (!b || !b->state || !rep) return set_error(APX_E_INVALID, "batch_results: null pointer");
   if (done) hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, stream, done);
   hipLaunchKernelGGL(results_kernel, dim3((b->ngal + 255) / 256), dim3(256), 0, stream, *b, rep,
-                     chi2, status, iters, done);
+                     chi2, status, iters, fallback, done);
   return check_launch("batch_results");
 }
 

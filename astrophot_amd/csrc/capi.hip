@@ -124,7 +124,13 @@ double apx_sersic_light_fraction(double n, double r_over_re) {
 }
 
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream) {
-  return apx::batch_start(batch, rep0, static_cast<hipStream_t>(stream));
+  return apx::batch_start(batch, rep0, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int apx_batch_start_fallback(const apx_batch* batch, const double* rep0, const double* fallback,
+                             void* stream) {
+  if (!fallback) return apx::set_error(APX_E_INVALID, "batch_start_fallback: null fallback");
+  return apx::batch_start(batch, rep0, fallback, static_cast<hipStream_t>(stream));
 }
 
 int apx_batch_tick(const apx_batch* batch, void* stream) {
@@ -132,8 +138,8 @@ int apx_batch_tick(const apx_batch* batch, void* stream) {
 }
 
 int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
-                      int32_t* iterations, int32_t* done, void* stream) {
-  return apx::batch_results(batch, rep, chi2, status, iterations, done,
+                      int32_t* iterations, int32_t* fallback, int32_t* done, void* stream) {
+  return apx::batch_results(batch, rep, chi2, status, iterations, fallback, done,
                             static_cast<hipStream_t>(stream));
 }
 

@@ -47,7 +47,7 @@ int batch_render(const apx_batch* b, const double* rep, float* out, int32_t part
                  hipStream_t stream);
 int batch_jacobian(const apx_batch* b, const double* rep, float* out, int32_t parts,
                  hipStream_t stream);
-int batch_start(const apx_batch* b, const double* rep0, hipStream_t stream);
+int batch_start(const apx_batch* b, const double* rep0, const double* fallback, hipStream_t stream);
 int batch_tick(const apx_batch* b, hipStream_t stream);
 int batch_moment_init(const apx_batch* b, double* rep, hipStream_t stream);
 double sersic_light_fraction(double n, double r_over_re);
@@ -67,6 +67,6 @@ int adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, floa
          float eps, int step, uint16_t* wb, hipStream_t stream);
 int asinh_input(const float* x, const float* sigma, int Bn, int npx, uint16_t* out, hipStream_t stream);
 int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,
-                  int32_t* done, hipStream_t stream);
+                  int32_t* fallback, int32_t* done, hipStream_t stream);
 
 }  // namespace apx

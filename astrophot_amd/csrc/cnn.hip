@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
   __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDK];  // Bs[n][k]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   v4f acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -255,8 +255,8 @@ int gemm_bf16(int M, int N, int K, const uint16_t* A, int64_t sam, int64_t sak, 
   if (M < 0 || N < 0 || K < 0 || !A || !B || !C || ldc < N || splits < 1)
     return set_error(APX_E_INVALID, "gemm_bf16: bad arguments");
   if (M == 0 || N == 0) return APX_OK;
-  const dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, splits);
-  if (grid.y > 65535 || splits > 65535) return set_error(APX_E_LIMIT, "gemm_bf16: M or splits too large");
+  const dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, splits);  // M tiles on x (up to 2^31 - 1)
+  if (grid.y > 65535 || splits > 65535) return set_error(APX_E_LIMIT, "gemm_bf16: N or splits too large");
   // k range per split: a multiple of the k-step
   const int kchunk = splits == 1 ? std::max(K, 1) : ((K + splits - 1) / splits + BK - 1) / BK * BK;
   if (splits > 1) {

@@ -6,13 +6,15 @@
 //   PA      angular centre of mass of the edge-subtracted light
 //           (utils/angle_operations.py:35-56, galaxy_model_object.py:78-96)
 //   q       here: PSF-corrected second moments of the pixels above 3 sigma of
-//           the edge (the reference scans isophote amplitudes over
-//           q in [0.2, 0.9], galaxy_model_object.py:97-113)
-//   Re, n   here: the half-light radius and the R80 / R20 concentration of
-//           the elliptical growth curve, inverted through the Sersic growth
-//           curve (the reference fits the binned radial profile with
-//           Nelder-Mead, _shared_methods.py:71-164)
-//   Ie      from the total flux, Re, n and q (utils/conversions/functions.py:168-190)
+//           the edge near the centre (the reference scans isophote amplitudes
+//           over q in [0.2, 0.9], galaxy_model_object.py:97-113)
+//   n, Re,  a Sersic fit of the binned elliptical radial profile, log10 I(R),
+//   Ie      by a grid search over (n, Re) with log10 Ie by least squares (the
+//           reference fits the same binned profile with Nelder-Mead,
+//           _shared_methods.py:71-164); when fewer than 3 bins have S/N >= 3,
+//           the half-light radius and R80 / R20 concentration of the growth
+//           curve, inverted through the Sersic growth curve, and Ie from the
+//           total flux (utils/conversions/functions.py:168-190)
 // One workgroup per cutout.  The edge level is the mean of the border pixels
 // and its scatter their standard deviation (the reference takes the median
 // and half the 16-84 inter-percentile range).
@@ -58,7 +60,9 @@ __device__ __forceinline__ double block_sumd(double v, double* red) {
 __global__ __launch_bounds__(256) void moment_init_kernel(apx_batch b, ConcTable T, double* rep) {
   __shared__ float redf[4];
   __shared__ double redd[4];
-  __shared__ float hist[kNB + 1];
+  __shared__ float hist[kNB + 1], cnt[kNB + 1], yk[kNB + 1], lrk[kNB + 1];
+  __shared__ float est[3], bres[4];
+  __shared__ int bidx[4], nvalid;
   __shared__ float cen[2];
   __shared__ int stop;
   for (int g = blockIdx.x; g < b.ngal; g += gridDim.x) {
@@ -165,14 +169,30 @@ __global__ __launch_bounds__(256) void moment_init_kernel(apx_batch b, ConcTable
     cs = block_sum(cs, redf);
     sn = block_sum(sn, redf);
     float ang = 0.5f * atan2f(sn, cs);  // angle of the light, mod pi
-    // ---- q: second moments along / across that angle, pixels above 3 sigma
+    // ---- q: second moments along / across that angle of the pixels above
+    // 3 sigma within 4 core radii (+2 px) of the centre; the core radius is the
+    // rms radius of the pixels above 5 sigma.  The radius cut keeps isolated
+    // noise pixels far out, whose r^2 lever arm would otherwise round the light.
     const float ca = cosf(ang), sa = sinf(ang);
     const float thr = 3.0f * scatter;
+    float c0 = 0.0f, c2 = 0.0f;
+    for (int p = threadIdx.x; p < n; p += 256) {
+      const float w = im[p] - edge;
+      if (!(w > 5.0f * scatter)) continue;
+      const float X = (p % nx) - pc, Y = (p / nx) - pr;
+      c0 += w;
+      c2 += w * (X * X + Y * Y);
+    }
+    c0 = block_sum(c0, redf);
+    c2 = block_sum(c2, redf);
+    const float rcore = c0 > 0.0f ? sqrtf(c2 / c0) : 0.125f * (float)min(nx, ny);
+    const float rlim2 = (4.0f * rcore + 2.0f) * (4.0f * rcore + 2.0f);
     float mu = 0.0f, mv = 0.0f, mw = 0.0f;
     for (int p = threadIdx.x; p < n; p += 256) {
       const float w = im[p] - edge;
       if (!(w > thr)) continue;
       const float X = (p % nx) - pc, Y = (p / nx) - pr;
+      if (X * X + Y * Y > rlim2) continue;
       const float u = ca * X + sa * Y, v = -sa * X + ca * Y;
       mu += w * u * u;
       mv += w * v * v;
@@ -181,21 +201,24 @@ __global__ __launch_bounds__(256) void moment_init_kernel(apx_batch b, ConcTable
     mu = block_sum(mu, redf);
     mv = block_sum(mv, redf);
     mw = block_sum(mw, redf);
+    // half the PSF second moment is removed: the thresholded moments see
+    // only the part of the PSF wings above the threshold
     float q = 0.7f;
     if (mw > 0.0f) {
-      const float a2 = fmaxf(mu / mw - psf2, 0.25f), b2 = fmaxf(mv / mw - psf2, 0.25f * 0.04f);
+      const float a2 = fmaxf(mu / mw - 0.5f * psf2, 0.25f);
+      const float b2 = fmaxf(mv / mw - 0.5f * psf2, 0.04f * a2);
       q = sqrtf(b2 / a2);
     }
     if (q > 1.0f) {  // the light is wider across the angle: swap the axes
       q = 1.0f / q;
       ang += 0.5f * (float)kPi;
     }
-    q = fminf(fmaxf(q, 0.1f), 0.95f);
+    q = fminf(fmaxf(q, 0.2f), 0.95f);
     const float cu = cosf(ang), su = sinf(ang);
     // ---- growth curve in elliptical radius
     const float rmax = 0.5f * sqrtf((float)(nx * nx + ny * ny)) + 1.0f;
     const float lstep = logf(rmax / kR0) / kNB;
-    for (int k = threadIdx.x; k <= kNB; k += 256) hist[k] = 0.0f;
+    for (int k = threadIdx.x; k <= kNB; k += 256) hist[k] = cnt[k] = 0.0f;
     __syncthreads();
     for (int p = threadIdx.x; p < n; p += 256) {
       const float X = (p % nx) - pc, Y = (p / nx) - pr;
@@ -203,6 +226,7 @@ __global__ __launch_bounds__(256) void moment_init_kernel(apx_batch b, ConcTable
       const float R = sqrtf(u * u + v * v);
       const int k = R < kR0 ? 0 : min(kNB, 1 + (int)(logf(R / kR0) / lstep));
       atomicAdd(&hist[k], im[p] - edge);
+      atomicAdd(&cnt[k], 1.0f);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -239,7 +263,90 @@ __global__ __launch_bounds__(256) void moment_init_kernel(apx_batch b, ConcTable
       const double bn = sersic_b((double)ns);
       const double flux_per_ie = 2.0 * kPi * (double)re * re * q * ns * exp(bn) * pow(bn, -2.0 * ns) *
                                  exp(lgamma(2.0 * ns));
-      const double ie = log10(fmax((double)tot, 1e-30) / flux_per_ie);
+      est[0] = re;
+      est[1] = ns;
+      est[2] = (float)log10(fmax((double)tot, 1e-30) / flux_per_ie);
+      // radial profile bins for the fit below: mean surface brightness with
+      // S/N >= 3, outside the PSF core when enough such bins remain
+      int nv = 0;
+      for (int pass = 0; pass < 2 && nv < 4; ++pass) {
+        nv = 0;
+        const float rcut = pass == 0 ? 1.5f * sqrtf(psf2) : 0.0f;
+        for (int k = 1; k <= kNB; ++k) {
+          const float rk = kR0 * expf(lstep * (k - 0.5f));
+          if (cnt[k] < 4.0f || rk < rcut) continue;
+          const float Ik = hist[k] / cnt[k];
+          if (!(Ik > 3.0f * scatter / sqrtf(cnt[k]))) continue;
+          yk[nv] = log10f(Ik);
+          lrk[nv] = logf(rk);
+          ++nv;
+        }
+      }
+      nvalid = nv;
+    }
+    __syncthreads();
+    // ---- Sersic profile fit of log10 I(R) over an (n, Re) grid, log10 Ie
+    // by least squares (the reference fits the same binned profile with
+    // Nelder-Mead, _shared_methods.py:126-164)
+    constexpr int kNRe = 48;
+    const int nv = nvalid;
+    float best = INFINITY;
+    int bi = -1;
+    if (nv >= 3) {
+      const float lre0 = logf(0.5f), lre1 = logf(4.0f * rmax);
+      for (int c = threadIdx.x; c < kNT * kNRe; c += 256) {
+        const float nn = T.n[c / kNRe];
+        if (nn < 0.4f || nn > 7.5f) continue;
+        const float lre = lre0 + (lre1 - lre0) * (float)(c % kNRe) / (kNRe - 1);
+        const float bn = (float)sersic_b((double)nn), inv = 1.0f / nn;
+        float s1 = 0.0f, s2 = 0.0f;
+        for (int k = 0; k < nv; ++k) {
+          const float m = -bn * (__expf((lrk[k] - lre) * inv) - 1.0f) * 0.43429448f;
+          const float d = yk[k] - m;
+          s1 += d;
+          s2 += d * d;
+        }
+        const float res = s2 - s1 * s1 / (float)nv;
+        if (res < best) {
+          best = res;
+          bi = c;
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const float ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob < best || (ob == best && oi >= 0 && (bi < 0 || oi < bi))) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if ((threadIdx.x & 63) == 0) {
+      bres[threadIdx.x >> 6] = best;
+      bidx[threadIdx.x >> 6] = bi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float re = est[0], ns = est[1];
+      double ie = est[2];
+      int c = -1;
+      float bb = INFINITY;
+      for (int w = 0; w < 4; ++w)
+        if (bidx[w] >= 0 && (bres[w] < bb || (bres[w] == bb && bidx[w] < c))) {
+          bb = bres[w];
+          c = bidx[w];
+        }
+      if (c >= 0) {
+        ns = T.n[c / kNRe];
+        const float lre0 = logf(0.5f), lre1 = logf(4.0f * rmax);
+        re = expf(lre0 + (lre1 - lre0) * (float)(c % kNRe) / (kNRe - 1));
+        const float bn = (float)sersic_b((double)ns), inv = 1.0f / ns;
+        float s1 = 0.0f;
+        for (int k = 0; k < nv; ++k) s1 += yk[k] + bn * (__expf((lrk[k] - logf(re)) * inv) - 1.0f) * 0.43429448f;
+        // log10 of the surface brightness at Re, per unit plane area
+        ie = (double)(s1 / (float)nv) - log10(fabs(b.P[0] * b.P[3] - b.P[1] * b.P[2]));
+      }
       // plane coordinates (pixel i at origin + i + 0.5 for the batch's pixel matrix)
       const double px = (double)pc + 0.5, py = (double)pr + 0.5;
       const double cx = G.origin[0] + b.P[0] * px + b.P[1] * py;

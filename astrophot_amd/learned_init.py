@@ -107,32 +107,19 @@ def cutout_images(bf: BatchFit) -> torch.Tensor:
 
 
 def learned_fit(bf: BatchFit, net: cnn.InitNet, sigma: torch.Tensor) -> Dict:
-    """LM of every cutout from the network's proposal, with the moment-init
-    fallback for the fits that do not converge.  Returns the merged results,
-    the per-path results and timings."""
+    """LM of every cutout from the network's proposal; a run that fails
+    (status 3 or 4) restarts once, inside the same batched pass, from the
+    moment-based start (the fallback and the judge)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     v = net.predict(cutout_images(bf), sigma)
     origin = np.array([[g.origin[0], g.origin[1]] for g in bf.gal_host])
     v[:, :2] += origin
     rep_c = torch.as_tensor(values_to_rep(v), dtype=torch.float64, device=bf.data.device)
-    res_c = bf.fit(rep_c)
-    fail = ~np.isin(res_c["status"], CONVERGED)
-    res = {k: (v_.copy() if isinstance(v_, np.ndarray) else v_) for k, v_ in res_c.items()}
-    res_f = None
-    if fail.any():
-        rep_m = bf.moment_init()
-        mask = torch.as_tensor(fail, device=rep_m.device)
-        rep_f = torch.where(mask[:, None], rep_m, torch.full_like(rep_m, float("nan")))
-        res_f = bf.fit(rep_f)
-        for k in ("rep", "values", "chi2", "status"):
-            res[k][fail] = res_f[k][fail]
-        res["iterations"] = res_c["iterations"] + np.where(fail, res_f["iterations"], 0)
+    res = bf.fit(rep_c, fallback=bf.moment_init())
     torch.cuda.synchronize()
     res["seconds"] = time.perf_counter() - t0
-    res["fallback"] = fail
     res["cnn_values"] = v
-    res["ticks"] = res_c["ticks"] + (res_f["ticks"] if res_f else 0)
     return res
 
 

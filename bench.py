@@ -1,20 +1,22 @@
 """Benchmark of the image-model hot path on one MI355X.
 
-Workload (N=1): 4096 independent Sersic cutouts (side log-uniform 32-256 px,
-n ~ U(0.5, 6), Re log-uniform 2-40 px, q ~ U(0.2, 1), PA ~ U(0, pi), S/N
-log-uniform 10-1000, Gaussian PSF sigma ~ U(1, 3) in a 4-sigma box), each
-fitted by AstroPhot's Levenberg-Marquardt (psf_mode "full", threshold
-supersampling, defaults of fit/lm.py) from a +-10 % perturbed start.  One step
-= fitting all 4096 cutouts to convergence with the batched device-resident LM
-(HIP-graph ticks).  value = LM fits/s.
+Headline workload (N=1): 65536 independent 64 x 64 Sersic cutouts (n ~ U(0.5,
+6), Re log-uniform 1.5-16 px, q ~ U(0.2, 1), PA ~ U(0, pi), S/N log-uniform
+10-1000, Gaussian PSF sigma ~ U(1, 3) in a 4-sigma box, seeded), each fitted
+by AstroPhot's Levenberg-Marquardt (psf_mode "full", threshold supersampling,
+defaults of fit/lm.py).  One step = the learned initialiser's proposal for
+every cutout, the batched device-resident LM (HIP-graph ticks), and LM from
+the moment-based start for the fits that do not converge.  value = LM fits/s.
 
-Beside it: the plain single-galaxy GPU path (astrophot_amd.fit.LM, one cutout
-at a time) on a subset, with the batched results checked against it (matched
-chi^2 and parameters); the batched convolution kernel's roofline measured live
-with HIP events; the float64 oracle LM on the host cores as the CPU baseline;
-and the 512x512 bulge+disk render (render Mpix/s, Jacobian evals/s).
+Beside it: the same batch from the moment-based start alone (fallback rate,
+LM iterations saved); the plain single-galaxy GPU path (astrophot_amd.fit.LM,
+one cutout at a time, from the moment start) on a subset with per-fit
+agreement; the dominant kernel's roofline measured live with HIP events; the
+float64 oracle LM on the host cores as the CPU baseline (same cutouts, same
+start); the 4096-cutout 32-256 px population from a perturbed truth; and the
+512x512 bulge+disk render (render Mpix/s, Jacobian evals/s).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ngal 4096] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ngal4 65536] [--no-cpu] [--no-c1] [--no-c2]
 """
 
 from __future__ import annotations
@@ -145,7 +147,7 @@ def plain_fits(pop, cuts, data, var, start, idx):
     return out, time.perf_counter() - t0
 
 
-def cpu_baseline(pop, data, var, start, budget_s=20.0):
+def cpu_baseline(pop, data, var, start, budget_s=20.0, start_label="the moment-based initialiser's start"):
     """float64 oracle LM (the reference algorithm restated, FFT convolution)
     on the host cores, cutout after cutout until ~budget_s."""
     from oracle import hotpath as H
@@ -167,86 +169,51 @@ def cpu_baseline(pop, data, var, start, budget_s=20.0):
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "fits/s", "cores": ncores, "kind": "port",
             "sample": f"the first {n} cutouts of the seeded population fitted one after another by the float64 "
-                      f"oracle LM (torch CPU, FFT conv, {npx} px in total) in {dt:.1f} s; the population's mean "
+                      f"oracle LM (torch CPU, FFT conv, {npx} px in total) from {start_label} in {dt:.1f} s; "
+                      f"the population's mean "
                       f"cutout is {float(np.mean(pop.sides.astype(float) ** 2)):.0f} px"}
 
 
-def main():
-    ap_ = argparse.ArgumentParser()
-    ap_.add_argument("--gpus", type=int, default=1)
-    ap_.add_argument("--steps", type=int, default=3)
-    ap_.add_argument("--warmup", type=int, default=1)
-    ap_.add_argument("--ngal", type=int, default=4096)
-    ap_.add_argument("--plain", type=int, default=24, help="cutouts fitted by the plain single-galaxy path")
-    ap_.add_argument("--no-cpu", action="store_true")
-    ap_.add_argument("--no-c1", action="store_true")
-    args = ap_.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the fits are independent "
-              f"and would shard by cutout, see DESIGN.md)", file=sys.stderr)
-        return 2
-
-    from astrophot_amd.batch import BatchFit, values_to_rep
-
-    pop, cuts, data, var, start = c2_problem(args.ngal)
-    bf = BatchFit(cuts, data=data, variance=var)
-    rep0 = torch.tensor(values_to_rep(start), dtype=torch.float64, device="cuda")
-    for _ in range(args.warmup):
-        res = bf.fit(rep0)
+def timed_events(fn, n=10, skip=2):
+    """Mean duration (us) of fn() over n calls after skip, with HIP events on the launch stream."""
+    evs = []
+    for _ in range(n):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        evs.append((a, b))
     torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    s.record()
-    for _ in range(args.steps):
-        res = bf.fit(rep0)
-    e.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = s.elapsed_time(e) / args.steps
-    fits_per_s = args.ngal / (ms * 1e-3)
-    status = {int(k): int(v) for k, v in zip(*np.unique(res["status"], return_counts=True))}
+    return 1e3 * float(np.mean([a.elapsed_time(b) for a, b in evs[skip:]]))
 
-    # dominant kernel: the Jacobian-stage convolution (value with K, dK/dcx,
-    # dK/dcy and the five partial planes with K: 8 kernel passes per output
-    # pixel) of every cutout, timed live with HIP events on the launch stream;
-    # the value convolution alone is reported beside it
-    truth_rep = torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda")
+
+def roofline(bf, truth_rep, label):
+    """The dominant kernel: the Jacobian-stage convolution (value with K,
+    dK/dcx, dK/dcy and the five partial planes with K: 8 kernel passes per
+    output pixel) of every cutout, timed live; the value convolution beside it."""
     out = torch.zeros(bf.npix, dtype=torch.float32, device="cuda")
-
-    def timed(fn, n=10, skip=2):
-        evs = []
-        for _ in range(n):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            fn()
-            b.record()
-            evs.append((a, b))
-        torch.cuda.synchronize()
-        return 1e3 * float(np.mean([a.elapsed_time(b) for a, b in evs[skip:]]))
-
     bf.jacobian(truth_rep, out=out, parts=1)
-    jac_us = timed(lambda: bf.jacobian(truth_rep, out=out, parts=2))
+    jac_us = timed_events(lambda: bf.jacobian(truth_rep, out=out, parts=2))
     bf.render(truth_rep, out=out, parts=1)
-    val_us = timed(lambda: bf.render(truth_rep, out=out, parts=2))
+    val_us = timed_events(lambda: bf.render(truth_rep, out=out, parts=2))
     vflops = bf.conv_flops()
     jflops = 8.0 * vflops
     achieved = jflops / (jac_us * 1e-6)
-    roof = {"bound": "valu", "kernel": "conv_kernel<1> (batched Jacobian-stage convolution, all 4096 cutouts)",
+    return {"bound": "valu", "kernel": f"conv_kernel<1> (batched Jacobian-stage convolution, {label})",
             "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32, "traffic": None, "algorithmic_per_launch": jflops,
             "avg_launch_us": jac_us,
             "value_conv": {"kernel": "conv_kernel<0>", "algorithmic_per_launch": vflops, "avg_launch_us": val_us,
                            "achieved": vflops / (val_us * 1e-6) / 1e12}}
 
-    # plain single-galaxy GPU path on a subset, and agreement with the batch
-    idx = list(range(min(args.plain, args.ngal)))
-    plain, t_plain = plain_fits(pop, cuts, data, var, start, idx)
-    plain_fps = len(idx) / t_plain
+
+def agreement(values, chi2, plain, idx):
+    """Per-fit agreement of batched results with the plain path's (params 1e-4, chi^2 1e-6)."""
     agree = {"fits": len(idx), "matched": 0, "param_rtol": 1e-4, "chi2_rtol": 1e-6,
              "max_param_rel_matched": 0.0, "max_chi2_rel_matched": 0.0, "unmatched": []}
     for k, (v, c2, msg) in zip(idx, plain):
-        perr = float(np.max(np.abs(res["values"][k] - v) / np.maximum(np.abs(v), 1.0)))
-        cerr = (float(res["chi2"][k]) - c2) / c2
+        perr = float(np.max(np.abs(values[k] - v) / np.maximum(np.abs(v), 1.0)))
+        cerr = (float(chi2[k]) - c2) / c2
         if perr <= agree["param_rtol"] and abs(cerr) <= agree["chi2_rtol"]:
             agree["matched"] += 1
             agree["max_param_rel_matched"] = max(agree["max_param_rel_matched"], perr)
@@ -255,28 +222,129 @@ def main():
             # signed: negative means the batched fit ended at the lower chi^2
             agree["unmatched"].append({"cutout": k, "param_rel": perr, "chi2_rel_batched_minus_plain": cerr,
                                        "q_fit": float(v[2]), "plain_message": msg[:40]})
+    return agree
 
-    line = {
-        "metric": METRIC, "value": fits_per_s, "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "C2: 4096 Sersic cutouts (side 32-256 px log-uniform), per-cutout Gaussian PSF, "
-                               "batched device-resident LM from a +-10% start; one step = all cutouts fitted",
-                   "ngal": args.ngal, "total_pixels": int(bf.npix), "seed": SEED, "init": "perturbed truth +-10%",
-                   "parallelism": "single GPU"},
+
+def c4_leg(args):
+    """Batched + learned-init LM of the 64 x 64 population: the network's
+    proposal for every cutout, LM, and LM from the moment-based start for the
+    fits that do not converge; beside it the moment-start batch and the plain
+    single-galaxy path (from the same moment start)."""
+    from astrophot_amd import learned_init as LI
+    from astrophot_amd.batch import BatchFit, rep_to_values, values_to_rep
+
+    n = args.ngal4
+    pop, cuts, data, var, sig = LI.c4_problem(n, SEED + 4)
+    bf = BatchFit(cuts, data=data, variance=var)
+    net = LI.load_net()
+    for _ in range(args.warmup):
+        LI.learned_fit(bf, net, sig)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = LI.learned_fit(bf, net, sig)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    rm = LI.moment_fit(bf)
+    rel = (res["chi2"] - rm["chi2"]) / rm["chi2"]
+    conv_l = np.isin(res["status"], LI.CONVERGED)
+    conv_m = np.isin(rm["status"], LI.CONVERGED)
+    # plain single-galaxy GPU path (astrophot_amd.fit.LM) from the moment start
+    idx = list(range(min(args.plain, n)))
+    start = rep_to_values(bf.moment_init().cpu().numpy())
+    plain, t_plain = plain_fits(pop, cuts, data, var, start, idx)
+    plain_fps = len(idx) / t_plain
+    roof = roofline(bf, torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda"),
+                    f"all {n} cutouts")
+    return {
+        "fits_per_s": n / sec, "seconds_per_step": sec, "ngal": n, "total_pixels": int(bf.npix),
+        "fallback_rate": float(res["fallback"].mean()),
+        "mean_iterations": float(res["iterations"].mean()),
+        "moment_start": {"fits_per_s": n / rm["seconds"], "mean_iterations": float(rm["iterations"].mean()),
+                         "converged": float(conv_m.mean())},
+        "iterations_saved_per_fit": float(rm["iterations"].mean() - res["iterations"].mean()),
+        "converged": float(conv_l.mean()),
+        "chi2_vs_moment_start": {"matched_1e-6": float((np.abs(rel) <= 1e-6).mean()),
+                                 "learned_lower": float((rel < -1e-6).mean()),
+                                 "learned_higher": float((rel > 1e-6).mean())},
+        "status_counts": {int(k): int(v) for k, v in zip(*np.unique(res["status"], return_counts=True))},
+        "plain_gpu": {"fits_per_s": plain_fps, "cutouts": len(idx), "seconds": t_plain,
+                      "start": "moment-based initialiser"},
+        "batched_learned_vs_plain": (n / sec) / plain_fps,
+        "batched_vs_plain_agreement": agreement(res["values"], res["chi2"], plain, idx),
         "roofline": roof,
-        "batched": {"fits_per_s": fits_per_s, "ticks": int(res["ticks"]), "status_counts": status,
-                    "mean_iterations": float(np.mean(res["iterations"]))},
-        "plain_gpu": {"fits_per_s": plain_fps, "cutouts": len(idx), "seconds": t_plain},
-        "batched_vs_plain": fits_per_s / plain_fps,
-        "batched_vs_plain_agreement": agree,
-        "wall_s_timed_region": wall,
+    }, (pop, data, var, start)
+
+
+def c2_leg(args):
+    """The 4096-cutout population (side 32-256 px) fitted from the perturbed truth."""
+    from astrophot_amd.batch import BatchFit, values_to_rep
+
+    pop, cuts, data, var, start = c2_problem(args.ngal)
+    bf = BatchFit(cuts, data=data, variance=var)
+    rep0 = torch.tensor(values_to_rep(start), dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        res = bf.fit(rep0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = bf.fit(rep0)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    idx = list(range(min(args.plain, args.ngal)))
+    plain, t_plain = plain_fits(pop, cuts, data, var, start, idx)
+    out = {
+        "workload": "4096 Sersic cutouts (side 32-256 px log-uniform), per-cutout Gaussian PSF, from a +-10% "
+                    "perturbed truth", "ngal": args.ngal, "total_pixels": int(bf.npix),
+        "fits_per_s": args.ngal / sec, "seconds_per_step": sec, "ticks": int(res["ticks"]),
+        "status_counts": {int(k): int(v) for k, v in zip(*np.unique(res["status"], return_counts=True))},
+        "mean_iterations": float(np.mean(res["iterations"])),
+        "plain_gpu": {"fits_per_s": len(idx) / t_plain, "cutouts": len(idx), "seconds": t_plain},
+        "batched_vs_plain": (args.ngal / sec) / (len(idx) / t_plain),
+        "batched_vs_plain_agreement": agreement(res["values"], res["chi2"], plain, idx),
+        "roofline": roofline(bf, torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda"),
+                             f"all {args.ngal} cutouts"),
     }
+    return out, (pop, data, var, start)
+
+
+def main():
+    ap_ = argparse.ArgumentParser()
+    ap_.add_argument("--gpus", type=int, default=1)
+    ap_.add_argument("--steps", type=int, default=3)
+    ap_.add_argument("--warmup", type=int, default=1)
+    ap_.add_argument("--ngal4", type=int, default=65536, help="64 x 64 cutouts of the learned-init workload")
+    ap_.add_argument("--ngal", type=int, default=4096, help="cutouts of the 32-256 px population")
+    ap_.add_argument("--plain", type=int, default=24, help="cutouts fitted by the plain single-galaxy path")
+    ap_.add_argument("--no-cpu", action="store_true")
+    ap_.add_argument("--no-c1", action="store_true")
+    ap_.add_argument("--no-c2", action="store_true")
+    args = ap_.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the fits are independent "
+              f"and would shard by cutout, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    c4, c4_data = c4_leg(args)
+    line = {
+        "metric": METRIC, "value": c4["fits_per_s"], "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * c4["seconds_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{args.ngal4} Sersic cutouts 64x64 (C2 priors with Re 1.5-16 px, S/N 10-1000, "
+                               f"Gaussian PSF sigma 1-3 px), batched LM from the learned initialiser's proposal "
+                               f"with moment-init fallback; one step = every cutout fitted",
+                   "ngal": args.ngal4, "seed": SEED + 4, "init": "CNN (BF16 MFMA), fallback moment-based",
+                   "parallelism": "single GPU"},
+        "roofline": c4.pop("roofline"),
+        "learned_init": c4,
+    }
+    if not args.no_c2:
+        line["c2"], _ = c2_leg(args)
     if not args.no_c1:
         line["c1"] = c1_render_leg()
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(pop, data, var, start)
-        line["gpu_vs_cpu"] = fits_per_s / line["cpu_baseline"]["value"]
+        line["cpu_baseline"] = cpu_baseline(*c4_data)
+        line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

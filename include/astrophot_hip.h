@@ -268,9 +268,9 @@ int apx_batch_jacobian(const apx_batch* batch, const double* rep, float* out, in
  * 176-233, utils/initialize/center.py:8-52), Galaxy_Model.initialize
  * (models/galaxy_model_object.py:56-113) and parametric_initialize
  * (models/_shared_methods.py:71-164).  Centre and PA follow the reference's
- * estimators; q, Re, n and Ie come from PSF-corrected moments and the
- * elliptical growth curve instead of the reference's isophote scan and
- * Nelder-Mead profile fit. */
+ * estimators; q comes from PSF-corrected second moments instead of the
+ * isophote scan, and n, Re, Ie from a grid-search Sersic fit of the binned
+ * radial profile instead of Nelder-Mead. */
 int apx_batch_moment_init(const apx_batch* batch, double* rep, void* stream);
 
 /* Fraction of a Sersic profile's light inside elliptical radius r (in units
@@ -282,15 +282,23 @@ double apx_sersic_light_fraction(double n, double r_over_re);
  * non-finite value is not run: it ends at once with status 5 (skipped). */
 int apx_batch_start(const apx_batch* batch, const double* rep0, void* stream);
 
+/* Start the fits from rep0 with a fallback start per fit (device, ngal x 7):
+ * a run that fails (status 3 or 4) restarts once from its fallback row within
+ * the same ticks, the way a caller would re-run LM from the initialise chain's
+ * start (fit/lm.py:427-493); a non-finite fallback row is not armed. */
+int apx_batch_start_fallback(const apx_batch* batch, const double* rep0, const double* fallback,
+                             void* stream);
+
 /* One LM tick of every unfinished fit. */
 int apx_batch_tick(const apx_batch* batch, void* stream);
 
 /* Results (device arrays): best state (res(), fit/base.py:104-117), its chi2
  * (reduced), status (0 running, 1 success, 2 success by immobility, 3 fail:
- * no improving step, 4 fail: max iterations, 5 skipped), iterations, and the
- * number of finished fits in done[0]. */
+ * no improving step, 4 fail: max iterations, 5 skipped), iterations (all
+ * runs), fallback (1 where the fallback start ran), and the number of
+ * finished fits in done[0].  Any output but rep may be null. */
 int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
-                      int32_t* iterations, int32_t* done, void* stream);
+                      int32_t* iterations, int32_t* fallback, int32_t* done, void* stream);
 
 /* Last error message of the calling thread. */
 const char* apx_last_error(void);

@@ -158,3 +158,31 @@ def test_batch_fit_matches_oracle_lm():
         err = np.max(np.abs(res["values"][k] - want) / np.maximum(np.abs(want), 1.0))
         assert err < PARAM_RTOL, (k, res["values"][k], want)
         assert abs(res["chi2"][k] - min(lm.loss_history)) / min(lm.loss_history) < CHI2_RTOL
+
+
+def test_fallback_restart_and_skipped_starts():
+    """apx_batch_start_fallback: with max_iter = 1 no run can converge (success
+    needs three history entries), so every fit ends its first run at max_iter,
+    restarts once from its fallback row, and ends exactly where a plain run
+    from that row ends.  A non-finite start is skipped (status 5) without
+    touching the other fits."""
+    cuts, starts = _problem(6, seed=21)
+    bf = BatchFit(cuts, max_iter=1)
+    rep_a = torch.tensor(values_to_rep(starts), dtype=torch.float64, device="cuda")
+    rep_b = rep_a.clone()
+    rep_b[:, 5] += 0.3
+    both = bf.fit(rep_a, fallback=rep_b, use_graph=False)
+    plain_b = bf.fit(rep_b, use_graph=False)
+    assert both["fallback"].all() and (both["iterations"] == 2).all(), (both["fallback"], both["iterations"])
+    assert (both["status"] == 4).all()
+    np.testing.assert_array_equal(both["rep"], plain_b["rep"])
+    np.testing.assert_array_equal(both["chi2"], plain_b["chi2"])
+    bf2 = BatchFit(cuts)
+    full = bf2.fit(rep_a, use_graph=False)
+    rep_nan = rep_a.clone()
+    rep_nan[2, 0] = float("nan")
+    part = bf2.fit(rep_nan, use_graph=False)
+    assert part["status"][2] == 5 and not part["fallback"].any()
+    keep = np.arange(6) != 2
+    np.testing.assert_array_equal(part["rep"][keep], full["rep"][keep])
+

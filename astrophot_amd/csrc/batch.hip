@@ -468,36 +468,40 @@ __global__ __launch_bounds__(256) void select_redo_kernel(apx_batch b, Scratch s
 }
 
 // exclusive prefix of n per-fit counts (one workgroup of 1024): off[0..n],
-// *total = off[n]; the counts are cleared after reading when `clear`
+// *total = off[n]; the counts are cleared after reading when `clear`.  Each
+// thread owns a contiguous run of ceil(n / 1024) counts: a serial prefix of
+// its run, one block-wide scan of the run totals, then the run's offsets.
 __global__ __launch_bounds__(1024) void scan_kernel(int n, int32_t* cnt, int32_t* off,
                                                     int32_t* total, int clear) {
   __shared__ int32_t wsums[16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int carry = 0;
-  for (int base = 0; base < n; base += 1024) {
-    const int g = base + threadIdx.x;
-    const int v = g < n ? cnt[g] : 0;
-    if (clear && g < n) cnt[g] = 0;
-    int x = v;
+  const int run = (n + 1023) / 1024;
+  const int g0 = min(n, (int)threadIdx.x * run), g1 = min(n, g0 + run);
+  int sum = 0;
+  for (int g = g0; g < g1; ++g) sum += cnt[g];
+  int x = sum;  // inclusive scan of the run totals
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsums[wv] = x;
-    __syncthreads();
-    int pre = 0, tot = 0;
-    for (int w = 0; w < 16; ++w) {
-      pre += w < wv ? wsums[w] : 0;
-      tot += wsums[w];
-    }
-    if (g < n) off[g] = carry + pre + x - v;
-    carry += tot;
-    __syncthreads();
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsums[wv] = x;
+  __syncthreads();
+  int pre = 0, tot = 0;
+  for (int w = 0; w < 16; ++w) {
+    pre += w < wv ? wsums[w] : 0;
+    tot += wsums[w];
+  }
+  int acc = pre + x - sum;  // exclusive prefix of this run
+  for (int g = g0; g < g1; ++g) {
+    const int v = cnt[g];
+    off[g] = acc;
+    acc += v;
+    if (clear) cnt[g] = 0;
   }
   if (threadIdx.x == 0) {
-    off[n] = carry;
-    *total = carry;
+    off[n] = tot;
+    *total = tot;
   }
 }
 

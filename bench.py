@@ -225,6 +225,13 @@ def agreement(values, chi2, plain, idx):
     return agree
 
 
+def learned_vs_plain(chi2, plain, idx):
+    """Signed relative chi^2 of the learned pipeline against the plain path (negative: learned lower)."""
+    rel = np.array([(float(chi2[k]) - c2) / c2 for k, (_, c2, _) in zip(idx, plain)])
+    return {"fits": len(idx), "matched_1e-6": int((np.abs(rel) <= 1e-6).sum()), "learned_lower": int((rel < -1e-6).sum()),
+            "learned_higher": int((rel > 1e-6).sum()), "max_rel_higher": float(max(rel.max(), 0.0))}
+
+
 def c4_leg(args):
     """Batched + learned-init LM of the 64 x 64 population: the network's
     proposal for every cutout, LM, and LM from the moment-based start for the
@@ -271,7 +278,10 @@ def c4_leg(args):
         "plain_gpu": {"fits_per_s": plain_fps, "cutouts": len(idx), "seconds": t_plain,
                       "start": "moment-based initialiser"},
         "batched_learned_vs_plain": (n / sec) / plain_fps,
-        "batched_vs_plain_agreement": agreement(res["values"], res["chi2"], plain, idx),
+        # correctness: the batched LM from the plain path's own (moment) start
+        "batched_vs_plain_agreement": agreement(rm["values"], rm["chi2"], plain, idx),
+        # outcome: where the learned pipeline ends against the plain path
+        "learned_vs_plain_chi2": learned_vs_plain(res["chi2"], plain, idx),
         "roofline": roof,
     }, (pop, data, var, start)
 

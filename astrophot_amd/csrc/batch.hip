@@ -35,6 +35,8 @@ constexpr int NP = APX_BATCH_NP;
 constexpr int kChunk = 256;
 constexpr int kPersist = 2048;
 constexpr int kNChan = 6;  // value + q, PA, n, Re, Ie partials
+constexpr int kTR = 16;      // output rows of a batch tile (64 x kTR pixels)
+constexpr int kMfmaMax = 35; // widest kernel the MFMA convolution takes
 
 enum : int32_t { PH_INIT = 0, PH_NEEDJ = 1, PH_SUB = 2, PH_DONE = 3 };
 enum : int32_t { ST_JAC = 0, ST_PROBE = 1, ST_TRIAL = 2, ST_ALL = 3, ST_ALL_JAC = 4 };
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void build_kernel(apx_batch b, Scratch s, int 
   const int nch = ((G.nx + 2 * bx) * (G.ny + 2 * by) + kChunk - 1) / kChunk;
   int base = atomicAdd(&s.cnt[1], nch);
   for (int c = 0; c < nch; ++c) s.chunk_list[base + c] = make_int2(g, c);
-  const int ntl = ((G.nx + 63) / 64) * ((G.ny + kRYF - 1) / kRYF);
+  const int ntl = ((G.nx + 63) / 64) * ((G.ny + kTR - 1) / kTR);
   base = atomicAdd(&s.cnt[2], ntl);
   for (int t = 0; t < ntl; ++t) s.tile_list[base + t] = make_int2(g, t);
 }
@@ -634,6 +636,8 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
 //   EPI_CHI2    sum W (Y - m)^2 of the tile                 (lm.py:290-292)
 enum : int { EPI_OUT = 0, EPI_NORMAL = 1, EPI_RPP = 2, EPI_CHI2 = 3 };
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // dynamic LDS of conv_kernel: the staged input tile, reused for the
 // cross-wave reduction of one kernel's outputs at a time
 __host__ __device__ inline int conv_stage_floats(int kmax) {
@@ -644,9 +648,50 @@ __host__ __device__ inline int conv_stage_floats(int kmax) {
 __host__ inline size_t conv_lds_bytes(int kmax) { return sizeof(float) * (size_t)conv_stage_floats(kmax); }
 constexpr size_t kConvStaticLds = sizeof(double) * 4 * 8;
 
-// After the cross-wave reduction each wave owns output rows w and w + 4 of
-// the tile (lane = column) for every kernel and channel, so the probe and
-// trial epilogues work from the two values each lane holds.
+// The probe (rpp) and trial (chi2) epilogues of one output pixel, summed into
+// rd[] in float64 (lm.py:276-292)
+template <int MODE>
+__device__ __forceinline__ void epilogue_pixel(const apx_batch& b, const GalState* st, int64_t p,
+                                               float m, double* rd) {
+  if (MODE == EPI_RPP) {
+    constexpr double d = 0.1;
+    float jr[NP];
+    double jh = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      jr[i] = b.jac[p * NP + i];
+      jh += (double)jr[i] * st->h[i];
+    }
+    const double wt = (double)b.weight[p];
+    const double v = (2.0 / d) * (wt * ((double)m - (double)b.model0[p]) / d - wt * jh);
+#pragma unroll
+    for (int i = 0; i < NP; ++i) rd[i] += (double)jr[i] * v;
+  } else if (MODE == EPI_CHI2) {
+    const double dd = (double)b.data[p] - (double)m;
+    rd[0] += (double)b.weight[p] * dd * dd;
+  }
+}
+
+// block sum of rd[0..W) into the tile's partial sums
+template <int W>
+__device__ __forceinline__ void tile_partial(double (&redd)[4][8], const double* rd, double* part,
+                                             int lane, int w) {
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const double v = wsum(rd[i]);
+    if (lane == 0) redd[w][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < W) {
+    const int i = threadIdx.x;
+    part[i] = redd[0][i] + redd[1][i] + redd[2][i] + redd[3][i];
+  }
+  __syncthreads();
+}
+
+// VALU convolution (kernels wider than kMfmaMax): the 64 x kTR tile in two
+// halves of kRYF rows; after the cross-wave reduction each wave owns output
+// rows w and w + 4 of a half (lane = column) for every kernel and channel
 template <int MODE>
 __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float* out) {
   extern __shared__ float smem[];
@@ -656,122 +701,205 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
   const int wu = __builtin_amdgcn_readfirstlane(w);
   constexpr int NKmax = JAC ? 3 : 1;
   constexpr int nchan = JAC ? kNChan : 1;
+  constexpr int W = MODE == EPI_RPP ? NP : 1;
   for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
     const int2 e = s.tile_list[k];
     const int g = __builtin_amdgcn_readfirstlane(e.x);
     const apx_batch_galaxy G = b.gal[g];
+    const GalState* st = st_of(b, g);
     const int nxw = G.nx + G.pw + 1, nyw = G.ny + G.ph + 1;  // cutout + 2 * border
     const int ntx = (G.nx + 63) / 64;
-    const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kRYF;
+    const int ox0 = (e.y % ntx) * 64;
     const int kh = G.ph + 2, kw = G.pw + 2;
     const int nT = fast_nt(kh), kts = fast_kts(kh), rows = 8 * nT, LW = 64 + kw - 1;
     const int64_t tile = G.tile_off + e.y;
     // output (ox, oy) sits at working (bx + ox, by + oy); taps span +-(k-1)/2 = border
     const float* kt = b.taps + G.tap_off;
     const int ox = ox0 + lane;
-    bool valid[2];
-    int64_t pp[2];
+    double rd[W];
 #pragma unroll
-    for (int sl = 0; sl < 2; ++sl) {
-      const int oy = oy0 + w + 4 * sl;
-      valid[sl] = ox < G.nx && oy < G.ny;
-      pp[sl] = G.pix_off + (int64_t)oy * G.nx + ox;
-    }
-    float mv[2] = {0.0f, 0.0f};
-    for (int c = 0; c < nchan; ++c) {
-      const float* plane = b.planes + (int64_t)c * b.nwork + G.work_off;
-      __syncthreads();
-      for (int q = threadIdx.x; q < rows * LW; q += 256) {
-        const int gy = oy0 + q / LW, gx = ox0 + q % LW;
-        smem[q] = (gy < nyw && gx < nxw) ? plane[(int64_t)gy * nxw + gx] : 0.0f;
+    for (int i = 0; i < W; ++i) rd[i] = 0.0;
+    for (int half = 0; half < kTR / kRYF; ++half) {
+      const int oy0 = (e.y / ntx) * kTR + half * kRYF;
+      bool valid[2];
+      int64_t pp[2];
+#pragma unroll
+      for (int sl = 0; sl < 2; ++sl) {
+        const int oy = oy0 + w + 4 * sl;
+        valid[sl] = ox < G.nx && oy < G.ny;
+        pp[sl] = G.pix_off + (int64_t)oy * G.nx + ox;
       }
-      __syncthreads();
-      const int nk = (JAC && c == 0) ? 3 : 1;
-      float acc[NKmax][kRYF];
-#pragma unroll
-      for (int kk = 0; kk < NKmax; ++kk)
-#pragma unroll
-        for (int r = 0; r < kRYF; ++r) acc[kk][r] = 0.0f;
-      conv_accumulate<NKmax>(smem, LW, lane, wu, nT, kw, kts, nk, kt, acc);
-#pragma unroll
-      for (int kk = 0; kk < NKmax; ++kk) {
-        if (kk >= nk) break;
+      for (int c = 0; c < nchan; ++c) {
+        const float* plane = b.planes + (int64_t)c * b.nwork + G.work_off;
         __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kRYF; ++r) smem[(w * kRYF + r) * 64 + lane] = acc[kk][r];
+        for (int q = threadIdx.x; q < rows * LW; q += 256) {
+          const int gy = oy0 + q / LW, gx = ox0 + q % LW;
+          smem[q] = (gy < nyw && gx < nxw) ? plane[(int64_t)gy * nxw + gx] : 0.0f;
+        }
         __syncthreads();
-        const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
+        const int nk = (JAC && c == 0) ? 3 : 1;
+        float acc[NKmax][kRYF];
 #pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-          const int r = w + 4 * sl;
-          float sum = 0.0f;
+        for (int kk = 0; kk < NKmax; ++kk)
 #pragma unroll
-          for (int ww = 0; ww < 4; ++ww) sum += smem[(ww * kRYF + r) * 64 + lane];
-          if (JAC) {
-            if (valid[sl]) {
+          for (int r = 0; r < kRYF; ++r) acc[kk][r] = 0.0f;
+        conv_accumulate<NKmax>(smem, LW, lane, wu, nT, kw, kts, nk, kt, acc);
+#pragma unroll
+        for (int kk = 0; kk < NKmax; ++kk) {
+          if (kk >= nk) break;
+          __syncthreads();
+#pragma unroll
+          for (int r = 0; r < kRYF; ++r) smem[(w * kRYF + r) * 64 + lane] = acc[kk][r];
+          __syncthreads();
+          const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
+#pragma unroll
+          for (int sl = 0; sl < 2; ++sl) {
+            const int r = w + 4 * sl;
+            float sum = 0.0f;
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) sum += smem[(ww * kRYF + r) * 64 + lane];
+            if (!valid[sl]) continue;
+            if (JAC) {
               if (col < 0) out[pp[sl]] = sum;
               else b.jac[pp[sl] * NP + col] = sum;
+            } else if (MODE == EPI_OUT) {
+              out[pp[sl]] = sum;
+            } else {
+              epilogue_pixel<MODE>(b, st, pp[sl], sum, rd);
             }
-          } else {
-            mv[sl] = sum;
-            if (MODE == EPI_OUT && valid[sl]) out[pp[sl]] = sum;
           }
         }
       }
     }
-    // ---- epilogue: the tile's partial sums, reduced in float64
-    double* part = s.part + tile * kPart;
-    if (MODE == EPI_OUT || MODE == EPI_NORMAL) continue;  // normal equations: normal_kernel
-    // per lane two pixels, summed in float64
-    constexpr int W = MODE == EPI_RPP ? NP : 1;
-    {
-      double rd[W];
-#pragma unroll
-      for (int i = 0; i < W; ++i) rd[i] = 0.0;
-      if (MODE == EPI_RPP) {
-        const GalState* st = st_of(b, g);
-        constexpr double d = 0.1;
-#pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-          if (!valid[sl]) continue;
-          float jr[NP];
-          double jh = 0.0;
-#pragma unroll
-          for (int i = 0; i < NP; ++i) {
-            jr[i] = b.jac[pp[sl] * NP + i];
-            jh += (double)jr[i] * st->h[i];
-          }
-          const double wt = (double)b.weight[pp[sl]];
-          const double v = (2.0 / d) * (wt * ((double)mv[sl] - (double)b.model0[pp[sl]]) / d - wt * jh);
-#pragma unroll
-          for (int i = 0; i < NP; ++i) rd[i] += (double)jr[i] * v;
-        }
-      } else {  // EPI_CHI2
-#pragma unroll
-        for (int sl = 0; sl < 2; ++sl) {
-          if (!valid[sl]) continue;
-          const double dd = (double)b.data[pp[sl]] - (double)mv[sl];
-          rd[0] += (double)b.weight[pp[sl]] * dd * dd;
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < W; ++i) {
-        const double v = wsum(rd[i]);
-        if (lane == 0) redd[w][i] = v;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < W) {
-      const int i = threadIdx.x;
-      part[i] = redd[0][i] + redd[1][i] + redd[2][i] + redd[3][i];
-    }
-    __syncthreads();
+    if (MODE == EPI_RPP || MODE == EPI_CHI2) tile_partial<W>(redd, rd, s.part + tile * kPart, lane, w);
   }
 }
 
-typedef float v4f __attribute__((ext_vector_type(4)));
+// ---------------------------------------------------------------------------
+// MFMA convolution (kernels up to kMfmaMax): each wave computes a 16 x 16
+// block of the 64 x kTR output tile with v_mfma_f32_16x16x4f32 as
+//   out[r][x] = sum_kx sum_j A_kx[x][j] B_kx[j][r],
+//   A_kx[x][j] = in[j][x + kx],  B_kx[j][r] = Kf[j - r][kx] (zero outside the kernel),
+// j over the staged input rows in k-steps of 4, grouped by 4.  A comes from the staged
+// input tile, B from a per-kernel tap table tab[kx][dd] = Kf[dd - 15][kx], both
+// in LDS.  Lane layout: A[l % 16][l / 16], B[l / 16][l % 16], and
+// D[4 (l / 16) + i][l % 16] in acc[i] (x = 4 (l / 16) + i, r = l % 16).
+// k-steps of 4 input rows come in groups of 4 (16 rows); the staged rows and
+// the tap table are padded with zeros to whole groups
+__host__ __device__ inline int mf_ng(int kh) { return (kTR + kh - 1 + 15) / 16; }  // groups per kx
+__host__ __device__ inline int mf_rows(int kh) { return 16 * mf_ng(kh); }
+__host__ __device__ inline int mf_lw(int kw) { return (64 + kw - 1) | 1; }  // odd pitch
+__host__ __device__ inline int mf_dd(int kh) { return mf_rows(kh) + 15; }
+__host__ inline size_t mfma_lds_bytes(int kmax, int nk) {
+  return sizeof(float) * ((size_t)mf_rows(kmax) * mf_lw(kmax) + (size_t)nk * kmax * mf_dd(kmax));
+}
 
-// per tile: J^T W J (MFMA 16x16x4 f32, 4 pixels per step, 128 pixels per
+template <int MODE>
+__global__ __launch_bounds__(256) void conv_mfma_kernel(apx_batch b, Scratch s, float* out) {
+  extern __shared__ float smem[];
+  __shared__ double redd[4][8];  // kConvStaticLds
+  constexpr bool JAC = MODE == EPI_NORMAL;
+  constexpr int nchan = JAC ? kNChan : 1;
+  constexpr int nkt = JAC ? 3 : 1;
+  constexpr int W = MODE == EPI_RPP ? NP : 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lm = lane & 15, lk = lane >> 4;
+  for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
+    const int2 e = s.tile_list[k];
+    const int g = __builtin_amdgcn_readfirstlane(e.x);
+    const apx_batch_galaxy G = b.gal[g];
+    const GalState* st = st_of(b, g);
+    const int nxw = G.nx + G.pw + 1, nyw = G.ny + G.ph + 1;  // cutout + 2 * border
+    const int ntx = (G.nx + 63) / 64;
+    const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kTR;
+    const int kh = G.ph + 2, kw = G.pw + 2;
+    const int ng = mf_ng(kh), LW = mf_lw(kw), DD = mf_dd(kh), kts = fast_kts(kh);
+    const int rows = mf_rows(kh), cols = 64 + kw - 1;
+    const int64_t tile = G.tile_off + e.y;
+    float* tin = smem;             // rows x LW
+    float* tab = smem + rows * LW; // [kk][kx][dd]
+    const float* kt = b.taps + G.tap_off;
+    __syncthreads();  // the previous tile's readers are done with LDS
+    for (int q = threadIdx.x; q < nkt * kw * DD; q += 256) {
+      const int dd = q % DD, kx = (q / DD) % kw, kk = q / (DD * kw);
+      const int a = dd - 15;  // Kf row; KT[kk][kx][u] = Kf[u - 7][kx]
+      tab[q] = (a >= 0 && a < kh) ? kt[((int64_t)kk * kw + kx) * kts + a + 7] : 0.0f;
+    }
+    // this lane's 4 output pixels: x = 16 w + 4 lk + i, row lm
+    const int oy = oy0 + lm;
+    bool valid[4];
+    int64_t pp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ox = ox0 + 16 * w + 4 * lk + i;
+      valid[i] = ox < G.nx && oy < G.ny;
+      pp[i] = G.pix_off + (int64_t)oy * G.nx + ox;
+    }
+    double rd[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) rd[i] = 0.0;
+    for (int c = 0; c < nchan; ++c) {
+      const float* plane = b.planes + (int64_t)c * b.nwork + G.work_off;
+      __syncthreads();
+      for (int q = threadIdx.x; q < rows * cols; q += 256) {
+        const int t = q / cols, u = q % cols;
+        const int gy = oy0 + t, gx = ox0 + u;
+        tin[t * LW + u] = (gy < nyw && gx < nxw) ? plane[(int64_t)gy * nxw + gx] : 0.0f;
+      }
+      __syncthreads();
+      const int nk = (JAC && c == 0) ? 3 : 1;
+      for (int kk = 0; kk < nk; ++kk) {
+        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float* ta = tin + lk * LW + 16 * w + lm;
+        const float* tb = tab + kk * kw * DD + lk - lm + 15;
+        // group q = (kx, jg): k-steps jj = 4 jg .. 4 jg + 3 of column kx; the
+        // next group's operands are loaded before this group's 4 MFMAs
+        const int nq = kw * ng;
+        float a[4], bb[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          a[t] = ta[4 * t * LW];
+          bb[t] = tb[4 * t];
+        }
+        for (int q = 0; q < nq; ++q) {
+          float an[4], bn[4];
+          const int qn = q + 1 < nq ? q + 1 : q;
+          const int kx = qn / ng, jg = qn % ng;
+          const float* pa = ta + kx + 16 * jg * LW;
+          const float* pb = tb + kx * DD + 16 * jg;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            an[t] = pa[4 * t * LW];
+            bn[t] = pb[4 * t];
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], bb[t], acc, 0, 0, 0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            a[t] = an[t];
+            bb[t] = bn[t];
+          }
+        }
+        const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (!valid[i]) continue;
+          if (JAC) {
+            if (col < 0) out[pp[i]] = acc[i];
+            else b.jac[pp[i] * NP + col] = acc[i];
+          } else if (MODE == EPI_OUT) {
+            out[pp[i]] = acc[i];
+          } else {
+            epilogue_pixel<MODE>(b, st, pp[i], acc[i], rd);
+          }
+        }
+      }
+    }
+    if (MODE == EPI_RPP || MODE == EPI_CHI2) tile_partial<W>(redd, rd, s.part + tile * kPart, lane, w);
+  }
+}
+
+// per tile: J^T W J (MFMA 16x16x4 f32, 4 pixels per step, 16 kTR pixels per
 // wave), J^T W (Y - m0) and chi2(m0), into the tile's partial sums
 __global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
   __shared__ double redH[4][256];
@@ -783,12 +911,12 @@ __global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
     const int2 e = s.tile_list[k];
     const apx_batch_galaxy G = b.gal[e.x];
     const int ntx = (G.nx + 63) / 64;
-    const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kRYF;
+    const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kTR;
     v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
     float gpart = 0.0f;
     double cacc = 0.0;
-    for (int stp = 0; stp < 32; ++stp) {
-      const int pt = wv * 128 + stp * 4 + kq;  // tile pixel
+    for (int stp = 0; stp < 16 * kTR / 4; ++stp) {
+      const int pt = wv * 16 * kTR + stp * 4 + kq;  // tile pixel
       const int px = ox0 + (pt & 63), py = oy0 + (pt >> 6);
       float a = 0.0f, wt = 0.0f, res = 0.0f;
       if (px < G.nx && py < G.ny) {
@@ -838,7 +966,7 @@ __global__ __launch_bounds__(256) void tile_reduce_kernel(apx_batch b, Scratch s
   for (int k = blockIdx.x; k < s.cnt[0]; k += gridDim.x) {
     const int g = s.gal_list[k];
     const apx_batch_galaxy G = b.gal[g];
-    const int ntl = ((G.nx + 63) / 64) * ((G.ny + kRYF - 1) / kRYF);
+    const int ntl = ((G.nx + 63) / 64) * ((G.ny + kTR - 1) / kTR);
     double acc = 0.0;
     if (e < W)
       for (int t = grp; t < ntl; t += 4) acc += s.part[(G.tile_off + t) * kPart + e];
@@ -1245,24 +1373,31 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
 // (`reduce`) the per-fit sums of the tile partials
 void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream,
                bool reduce) {
-  const size_t lds = conv_lds_bytes(b.max_kernel);
+  const bool mf = b.max_kernel <= kMfmaMax;  // MFMA convolution, else the VALU one
+  const size_t lds = mf ? mfma_lds_bytes(b.max_kernel, mode == EPI_NORMAL ? 3 : 1) : conv_lds_bytes(b.max_kernel);
   const int tc = (int)std::min<int64_t>(b.ntile, 2 * kPersist);
+#define APX_CONV(M)                                                                              \
+  do {                                                                                           \
+    if (mf) hipLaunchKernelGGL(conv_mfma_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out); \
+    else hipLaunchKernelGGL(conv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);        \
+  } while (0)
   switch (mode) {
     case EPI_OUT:
-      hipLaunchKernelGGL(conv_kernel<EPI_OUT>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      APX_CONV(EPI_OUT);
       return;
     case EPI_NORMAL:
-      hipLaunchKernelGGL(conv_kernel<EPI_NORMAL>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      APX_CONV(EPI_NORMAL);
       if (!reduce) return;
       hipLaunchKernelGGL(normal_kernel, dim3(tc), dim3(256), 0, stream, b, s);
       break;
     case EPI_RPP:
-      hipLaunchKernelGGL(conv_kernel<EPI_RPP>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      APX_CONV(EPI_RPP);
       break;
     default:
-      hipLaunchKernelGGL(conv_kernel<EPI_CHI2>, dim3(tc), dim3(256), lds, stream, b, s, out);
+      APX_CONV(EPI_CHI2);
       break;
   }
+#undef APX_CONV
   hipLaunchKernelGGL(tile_reduce_kernel, dim3(std::min(b.ngal, 2 * kPersist)), dim3(256), 0, stream,
                      b, s, mode);
 }
@@ -1301,10 +1436,10 @@ int batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* b) {
     work += nw;
     tap += 3 * (int64_t)(G.pw + 2) * fast_kts(G.ph + 2);
     chunk += (nw + kChunk - 1) / kChunk;
-    tile += (int64_t)((G.nx + 63) / 64) * ((G.ny + kRYF - 1) / kRYF);
+    tile += (int64_t)((G.nx + 63) / 64) * ((G.ny + kTR - 1) / kTR);
     kmax = std::max(kmax, std::max(G.ph, G.pw) + 2);
   }
-  if (conv_lds_bytes(kmax) + kConvStaticLds > 65536)
+  if ((kmax <= kMfmaMax ? mfma_lds_bytes(kmax, 3) : conv_lds_bytes(kmax)) + kConvStaticLds > 65536)
     return set_error(APX_E_LIMIT, "batch_plan: psf too large for the batched Jacobian convolution");
   if (work > (int64_t)INT32_MAX || chunk > (int64_t)INT32_MAX || tile > (int64_t)INT32_MAX)
     return set_error(APX_E_LIMIT, "batch_plan: batch too large");

@@ -318,6 +318,30 @@ def c2_leg(args):
     return out, (pop, data, var, start)
 
 
+def kernel_only(args):
+    """For counter collection (rocprofv3 --pmc): the headline batch's
+    Jacobian-stage convolution over every cutout 3 times, then a plain 1 GiB
+    device copy twice (the counters' calibration: a copy of known size)."""
+    from astrophot_amd import learned_init as LI
+    from astrophot_amd.batch import BatchFit, values_to_rep
+
+    pop, cuts, data, var, sig = LI.c4_problem(args.ngal4, SEED + 4)
+    bf = BatchFit(cuts, data=data, variance=var)
+    rep = torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda")
+    out = torch.zeros(bf.npix, dtype=torch.float32, device="cuda")
+    bf.jacobian(rep, out=out, parts=1)
+    for _ in range(3):
+        bf.jacobian(rep, out=out, parts=2)
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_only": True, "ngal": args.ngal4, "jacobian_conv_launches": 3,
+                      "copy_bytes": src.numel() * 4, "copies": 2}))
+    return 0
+
+
 def main():
     ap_ = argparse.ArgumentParser()
     ap_.add_argument("--gpus", type=int, default=1)
@@ -329,11 +353,15 @@ def main():
     ap_.add_argument("--no-cpu", action="store_true")
     ap_.add_argument("--no-c1", action="store_true")
     ap_.add_argument("--no-c2", action="store_true")
+    ap_.add_argument("--kernel-only", action="store_true",
+                     help="run only the roofline kernel (and a calibration copy) for counter collection")
     args = ap_.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the fits are independent "
               f"and would shard by cutout, see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.kernel_only:
+        return kernel_only(args)
 
     c4, c4_data = c4_leg(args)
     line = {

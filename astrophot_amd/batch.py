@@ -190,8 +190,8 @@ class BatchFit:
     def jacobian(self, rep, out=None, parts=3):
         """All cutouts rendered at ``rep`` into ``out`` (as render) and their
         Jacobians with respect to the 7 representation-space parameters,
-        returned as an (npix, 7) view of the batch's J buffer (pixel-major,
-        cutouts concatenated).  ``parts`` as in render.  The LM state of the
+        returned as an (npix, 7) view of the batch's J buffer (stored
+        parameter-major, 7 x npix, cutouts concatenated).  ``parts`` as in render.  The LM state of the
         fits is left untouched."""
         rep = torch.as_tensor(rep, dtype=torch.float64, device=self.data.device).contiguous()
         self._render_rep = rep
@@ -199,7 +199,7 @@ class BatchFit:
             out = torch.zeros(self.npix, dtype=torch.float32, device=self.data.device)
         L.check(self.lib.apx_batch_jacobian(ctypes.byref(self.b), rep.data_ptr(), out.data_ptr(), parts,
                                             L.stream()))
-        return out, self.jac.view(self.npix, NP)
+        return out, self.jac.view(NP, self.npix).t()
 
     def moment_init(self) -> torch.Tensor:
         """Moment-based starting parameters of every cutout from its data

@@ -5,7 +5,7 @@
 // lock-step "ticks".  A tick is a fixed launch sequence over device-built work
 // lists (so it can be captured in a HIP graph):
 //   Jacobian stage (fits starting a step):   render value + 5 partial planes,
-//       convolve with K, dK/dcx, dK/dcy -> model0 and J (pixel-major, 7 cols),
+//       convolve with K, dK/dcx, dK/dcy -> model0 and J (parameter-major, 7 x npix),
 //       J^T W J, J^T W (Y - m0), chi2(m0)                 (fit/lm.py:256-260)
 //   lm_begin: step bookkeeping                              (lm.py:261-267)
 //   lm_solve_h: damped float64 solve, probe point x + 0.1 h (lm.py:269-277)
@@ -659,7 +659,7 @@ __device__ __forceinline__ void epilogue_pixel(const apx_batch& b, const GalStat
     double jh = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
-      jr[i] = b.jac[p * NP + i];
+      jr[i] = b.jac[(int64_t)i * b.npix + p];
       jh += (double)jr[i] * st->h[i];
     }
     const double wt = (double)b.weight[p];
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
             if (!valid[sl]) continue;
             if (JAC) {
               if (col < 0) out[pp[sl]] = sum;
-              else b.jac[pp[sl] * NP + col] = sum;
+              else b.jac[(int64_t)col * b.npix + pp[sl]] = sum;
             } else if (MODE == EPI_OUT) {
               out[pp[sl]] = sum;
             } else {
@@ -886,7 +886,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(apx_batch b, Scratch s, 
           if (!valid[i]) continue;
           if (JAC) {
             if (col < 0) out[pp[i]] = acc[i];
-            else b.jac[pp[i] * NP + col] = acc[i];
+            else b.jac[(int64_t)col * b.npix + pp[i]] = acc[i];
           } else if (MODE == EPI_OUT) {
             out[pp[i]] = acc[i];
           } else {
@@ -924,7 +924,7 @@ __global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
         const float m = b.model0[p], y = b.data[p];
         wt = b.weight[p];
         res = y - m;
-        if (r16 < NP) a = b.jac[p * NP + r16];
+        if (r16 < NP) a = b.jac[(int64_t)r16 * b.npix + p];
         if (r16 == 0) {
           const double dd = (double)y - (double)m;
           cacc += (double)wt * dd * dd;

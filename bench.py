@@ -174,6 +174,24 @@ def cpu_baseline(pop, data, var, start, budget_s=20.0, start_label="the moment-b
                       f"cutout is {float(np.mean(pop.sides.astype(float) ** 2)):.0f} px"}
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "roofline_traffic.json")
+
+
+def measured_traffic(kernel: str, ngal: int):
+    """HBM bytes per launch of the roofline kernel from the rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/roofline_traffic.json
+    (calibrated against a device copy of known size), when they were taken on
+    the same kernel and workload size; else None."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or t.get("ngal") != ngal:
+        return None
+    return t
+
+
 def timed_events(fn, n=10, skip=2):
     """Mean duration (us) of fn() over n calls after skip, with HIP events on the launch stream."""
     evs = []
@@ -199,11 +217,15 @@ def roofline(bf, truth_rep, label):
     vflops = bf.conv_flops()
     jflops = 8.0 * vflops
     achieved = jflops / (jac_us * 1e-6)
-    return {"bound": "valu", "kernel": f"conv_kernel<1> (batched Jacobian-stage convolution, {label})",
+    kernel = "conv_mfma_kernel<1>" if bf.b.max_kernel <= 35 else "conv_kernel<1>"
+    t = measured_traffic(kernel, len(bf.gal_host))
+    return {"bound": "valu", "kernel": f"{kernel} (batched Jacobian-stage convolution, {label})",
             "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32, "traffic": None, "algorithmic_per_launch": jflops,
+            "frac": achieved / PEAK_FP32, "traffic": t["bytes_per_launch"] if t else None,
+            "traffic_detail": t, "algorithmic_per_launch": jflops,
             "avg_launch_us": jac_us,
-            "value_conv": {"kernel": "conv_kernel<0>", "algorithmic_per_launch": vflops, "avg_launch_us": val_us,
+            "value_conv": {"kernel": kernel.replace("<1>", "<0>"), "algorithmic_per_launch": vflops,
+                           "avg_launch_us": val_us,
                            "achieved": vflops / (val_us * 1e-6) / 1e12}}
 
 

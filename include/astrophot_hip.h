@@ -235,7 +235,8 @@ typedef struct apx_batch {
   float* model0;               /* npix: model at the current state */
   float* model1;               /* npix: spare model buffer (the LM ticks reduce their
                                   probe and trial renders in place, unstored) */
-  float* jac;                  /* npix x APX_BATCH_NP, pixel-major */
+  float* jac;                  /* APX_BATCH_NP x npix, parameter-major (column c of
+                                  J at jac + c * npix) */
   float* planes;               /* 6 x nwork */
   float* taps;                 /* ntap */
   void* state;                 /* ngal x apx_batch_state_bytes() */
@@ -254,8 +255,8 @@ int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int3
                      void* stream);
 
 /* Render every cutout at rep into out (npix) and its Jacobian with respect
- * to the 7 representation-space parameters into batch->jac (npix x 7,
- * pixel-major): the batched counterpart of Component_Model.jacobian(
+ * to the 7 representation-space parameters into batch->jac (7 x npix,
+ * parameter-major): the batched counterpart of Component_Model.jacobian(
  * as_representation=True) (models/_model_methods.py:260-347), the columns of
  * the J the LM's Jacobian stage forms (fit/lm.py:256-257).  parts as in
  * apx_batch_render.  Leaves the LM state of the fits untouched. */

@@ -18,7 +18,7 @@ from __future__ import annotations
 import ctypes
 import math
 from dataclasses import dataclass
-from typing import List, Optional, Sequence
+from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -53,7 +53,9 @@ class Batch(ctypes.Structure):
                 ("quad_level", ctypes.c_int32), ("gridding", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("max_iter", ctypes.c_int32), ("relative_tolerance", ctypes.c_double), ("L0", ctypes.c_double),
                 ("gal", _p), ("data", _p), ("weight", _p), ("psf", _p), ("model0", _p), ("model1", _p),
-                ("jac", _p), ("planes", _p), ("taps", _p), ("state", _p), ("scratch", _p)]
+                ("jac", _p), ("planes", _p), ("taps", _p), ("state", _p), ("scratch", _p),
+                ("conv_impl", ctypes.c_int32), ("conv_blocks", ctypes.c_int32), ("quad_blocks", ctypes.c_int32),
+                ("quad_group", ctypes.c_int32), ("chunk_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 SIGNATURES = {
@@ -207,6 +209,73 @@ class BatchFit:
         rep = torch.empty((len(self.gal_host), NP), dtype=torch.float64, device=self.data.device)
         L.check(self.lib.apx_batch_moment_init(ctypes.byref(self.b), rep.data_ptr(), L.stream()))
         return rep
+
+    # -- launch-shape autotuning ---------------------------------------------
+    def _time(self, fn, n=4, skip=1) -> float:
+        evs = []
+        for _ in range(n):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            evs.append((a, b))
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in evs[skip:]]))
+
+    def autotune(self, rep=None) -> Dict:
+        """Pick the launch shape of this batch by timing candidates on it (HIP
+        events on the launch stream), one stage at a time:
+          convolution  VALU or MFMA kernel x workgroup count, on the Jacobian-
+                       stage convolution of every cutout (the costliest launch);
+          quadrature   lanes per refined pixel (32 or 64) x workgroup count,
+          chunks       workgroup count of the per-chunk render kernels,
+                       both on the Jacobian-stage render planes.
+        The winners stay in the batch descriptor for every later render, Jacobian
+        and LM tick; returns them with the timings (ms)."""
+        if rep is None:
+            rep = getattr(self, "_render_rep", None)
+        if rep is None:
+            raise SpecificationConflict("autotune needs parameters to render at")
+        rep = torch.as_tensor(rep, dtype=torch.float64, device=self.data.device).contiguous()
+        out = torch.zeros(self.npix, dtype=torch.float32, device=self.data.device)
+        b = self.b
+        log = {"conv": {}, "quad": {}, "chunk": {}}
+        self.jacobian(rep, out=out, parts=1)
+        impls = [1] + ([2] if b.max_kernel <= 35 else [])
+        best = None
+        conv_grid = sorted({min(c, int(b.ntile)) for c in (1024, 2048, 4096, 8192, 16384)})  # launches clamp to ntile
+        for impl in impls:
+            for blocks in conv_grid:
+                b.conv_impl, b.conv_blocks = impl, blocks
+                ms = self._time(lambda: self.jacobian(rep, out=out, parts=2))
+                log["conv"][f"{'mfma' if impl == 2 else 'valu'}:{blocks}"] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, impl, blocks)
+        b.conv_impl, b.conv_blocks = best[1], best[2]
+        groups = [0] + ([64] if b.gridding * b.gridding <= 32 else [])
+        best = None
+        for grp in groups:
+            for blocks in (512, 1024, 2048, 4096):
+                b.quad_group, b.quad_blocks = grp, blocks
+                ms = self._time(lambda: self.jacobian(rep, out=out, parts=1))
+                log["quad"][f"{grp or (32 if b.gridding * b.gridding <= 32 else 64)}:{blocks}"] = ms
+                if best is None or ms < best[0]:
+                    best = (ms, grp, blocks)
+        b.quad_group, b.quad_blocks = best[1], best[2]
+        best = None
+        for blocks in sorted({min(c, int(b.nchunk)) for c in (2048, 4096, 8192, 16384)}):
+            b.chunk_blocks = blocks
+            ms = self._time(lambda: self.jacobian(rep, out=out, parts=1))
+            log["chunk"][str(blocks)] = ms
+            if best is None or ms < best[0]:
+                best = (ms, blocks)
+        b.chunk_blocks = best[1]
+        self.graph = None  # a captured tick holds the old launch shape
+        choice = {"conv_impl": "mfma" if b.conv_impl == 2 else "valu", "conv_blocks": b.conv_blocks,
+                  "quad_group": b.quad_group or (32 if b.gridding * b.gridding <= 32 else 64),
+                  "quad_blocks": b.quad_blocks, "chunk_blocks": b.chunk_blocks}
+        self.tuning = {"choice": choice, "timings_ms": log}
+        return self.tuning
 
     def conv_flops(self) -> float:
         """Algorithmic flops of one value convolution of every cutout:

@@ -1329,7 +1329,7 @@ QuadTab quad_table(const apx_batch& b) {
 template <int GS>
 void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac,
                       hipStream_t stream) {
-  const dim3 ig(kPersist / 2);
+  const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, false>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, false>), ig, dim3(256), 0, stream, b, s, t);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
@@ -1347,7 +1347,7 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
   hipLaunchKernelGGL(build_kernel, dim3(gb), dim3(256), 0, stream, b, s, stage);
   hipLaunchKernelGGL(prepare_kernel, dim3(std::min(b.ngal, 1024)), dim3(256), 0, stream, b, s, stage,
                      rep_override);
-  const int pc = (int)std::min<int64_t>(b.nchunk, 4 * kPersist);
+  const int pc = (int)std::min<int64_t>(b.nchunk, b.chunk_blocks > 0 ? b.chunk_blocks : 4 * kPersist);
   const QuadTab t = quad_table(b);
   if (jac) {
     hipLaunchKernelGGL(init_kernel<true>, dim3(pc), dim3(kChunk), 0, stream, b, s);
@@ -1358,14 +1358,14 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
   }
   // float64 redo of undecided selections, then the level-2/3 quadrature of
   // the deep list, then its undecided pixels in float64
-  const dim3 ig(kPersist / 2);
+  const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
                      s.cnt + 4, 1);
   if (jac) hipLaunchKernelGGL(select_redo_kernel<true>, ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL(select_redo_kernel<false>, ig, dim3(256), 0, stream, b, s, t);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_cnt, s.gal_off,
                      s.cnt + 3, 0);
-  if (t.gg <= 32) launch_integrate<32>(b, s, t, jac, stream);
+  if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32>(b, s, t, jac, stream);
   else launch_integrate<64>(b, s, t, jac, stream);
 }
 
@@ -1373,9 +1373,10 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
 // (`reduce`) the per-fit sums of the tile partials
 void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream,
                bool reduce) {
-  const bool mf = b.max_kernel <= kMfmaMax;  // MFMA convolution, else the VALU one
+  // MFMA convolution unless the caller asks for the VALU one or the PSF is too wide
+  const bool mf = b.conv_impl != 1 && b.max_kernel <= kMfmaMax;
   const size_t lds = mf ? mfma_lds_bytes(b.max_kernel, mode == EPI_NORMAL ? 3 : 1) : conv_lds_bytes(b.max_kernel);
-  const int tc = (int)std::min<int64_t>(b.ntile, 2 * kPersist);
+  const int tc = (int)std::min<int64_t>(b.ntile, b.conv_blocks > 0 ? b.conv_blocks : 2 * kPersist);
 #define APX_CONV(M)                                                                              \
   do {                                                                                           \
     if (mf) hipLaunchKernelGGL(conv_mfma_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out); \
@@ -1465,6 +1466,15 @@ static int check_batch(const apx_batch* b) {
       b->gridding > APX_MAX_GRIDDING || b->max_depth < 1 || b->max_depth > APX_MAX_DEPTH)
     return set_error(APX_E_LIMIT, "batch: integration settings out of range");
   if (b->max_kernel > kFastMax) return set_error(APX_E_LIMIT, "batch: psf too large");
+  if (b->conv_impl < 0 || b->conv_impl > 2 || (b->conv_impl == 2 && b->max_kernel > kMfmaMax))
+    return set_error(APX_E_INVALID, "batch: conv_impl must be 0, 1 or 2 (2 needs psf <= 33 px)");
+  if (b->quad_group != 0 && b->quad_group != 32 && b->quad_group != 64)
+    return set_error(APX_E_INVALID, "batch: quad_group must be 0, 32 or 64");
+  if (b->quad_group == 32 && b->gridding * b->gridding > 32)
+    return set_error(APX_E_INVALID, "batch: quad_group 32 needs gridding <= 5");
+  if (b->conv_blocks < 0 || b->quad_blocks < 0 || b->chunk_blocks < 0 || b->conv_blocks > (1 << 20) ||
+      b->quad_blocks > (1 << 20) || b->chunk_blocks > (1 << 20))
+    return set_error(APX_E_INVALID, "batch: launch sizes out of range");
   return APX_OK;
 }
 

@@ -241,6 +241,13 @@ typedef struct apx_batch {
   float* taps;                 /* ntap */
   void* state;                 /* ngal x apx_batch_state_bytes() */
   void* scratch;               /* apx_batch_scratch_bytes(batch) */
+  /* launch shape (0 = built-in default); set by the caller's autotuner */
+  int32_t conv_impl;           /* 1 = VALU convolution, 2 = MFMA (kernels <= 35 px) */
+  int32_t conv_blocks;         /* workgroups of the convolution launches */
+  int32_t quad_blocks;         /* workgroups of the level-2/3 quadrature launches */
+  int32_t quad_group;          /* lanes per refined pixel: 32 (gridding <= 5) or 64 */
+  int32_t chunk_blocks;        /* workgroups of the per-chunk render launches */
+  int32_t reserved;
 } apx_batch;
 
 /* Fill the offsets of gal[] (host array) and the totals of *batch. */

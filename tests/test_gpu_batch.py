@@ -197,3 +197,28 @@ def test_fallback_restart_and_skipped_starts():
     keep = np.arange(6) != 2
     np.testing.assert_array_equal(part["rep"][keep], full["rep"][keep])
 
+
+def test_launch_shapes_agree_and_autotune():
+    """Every launch shape the autotuner may pick (VALU or MFMA convolution,
+    32 or 64 lanes per refined pixel, workgroup counts) renders the same model
+    and Jacobian (float32 summation order only: 1e-6 of each column's max), and
+    autotune() leaves one of them in the descriptor."""
+    truths, shapes, rng = make_cutouts(5, seed=8)
+    cuts = [Cutout(np.zeros((s, s)), np.ones((s, s)), psf) for s, psf in shapes]
+    bf = BatchFit(cuts)
+    rep = torch.tensor(values_to_rep(np.stack(truths)), dtype=torch.float64, device="cuda")
+    ref_m, ref_j = [t.clone() for t in bf.jacobian(rep)]
+    for impl, grp, cb, qb, kb in ((1, 0, 0, 0, 0), (2, 64, 7, 3, 5), (1, 64, 1, 1, 1), (2, 0, 4096, 4096, 9)):
+        b = bf.b
+        b.conv_impl, b.quad_group, b.conv_blocks, b.quad_blocks, b.chunk_blocks = impl, grp, cb, qb, kb
+        m, j = bf.jacobian(rep)
+        assert float((m - ref_m).abs().max() / ref_m.abs().max()) < 1e-6, (impl, grp)
+        scale = ref_j.abs().amax(dim=0)
+        assert float(((j - ref_j).abs().amax(dim=0) / scale).max()) < 1e-6, (impl, grp)
+    tuning = bf.autotune(rep)
+    ch = tuning["choice"]
+    assert ch["conv_impl"] in ("mfma", "valu") and ch["quad_group"] in (32, 64)
+    assert ch["conv_blocks"] > 0 and ch["quad_blocks"] > 0 and ch["chunk_blocks"] > 0
+    m, j = bf.jacobian(rep)
+    assert float((m - ref_m).abs().max() / ref_m.abs().max()) < 1e-6
+

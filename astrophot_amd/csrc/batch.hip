@@ -852,33 +852,34 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(apx_batch b, Scratch s, 
         v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
         const float* ta = tin + lk * LW + 16 * w + lm;
         const float* tb = tab + kk * kw * DD + lk - lm + 15;
-        // group q = (kx, jg): k-steps jj = 4 jg .. 4 jg + 3 of column kx; the
-        // next group's operands are loaded before this group's 4 MFMAs
+        // group q = (kx, jg): k-steps jj = 4 jg .. 4 jg + 3 of column kx.  Two
+        // operand sets alternate: the loads of the next group are in flight
+        // while the MFMAs of the current one issue.
         const int nq = kw * ng;
-        float a[4], bb[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          a[t] = ta[4 * t * LW];
-          bb[t] = tb[4 * t];
-        }
-        for (int q = 0; q < nq; ++q) {
-          float an[4], bn[4];
-          const int qn = q + 1 < nq ? q + 1 : q;
-          const int kx = qn / ng, jg = qn % ng;
+        auto load = [&](int q, float (&a)[4], float (&bb)[4]) {
+          const int kx = q / ng, jg = q - kx * ng;
           const float* pa = ta + kx + 16 * jg * LW;
           const float* pb = tb + kx * DD + 16 * jg;
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
-            an[t] = pa[4 * t * LW];
-            bn[t] = pb[4 * t];
+            a[t] = pa[4 * t * LW];
+            bb[t] = pb[4 * t];
           }
+        };
+        float a0[4], b0[4], a1[4], b1[4];
+        load(0, a0, b0);
+        int q = 0;
+        for (; q + 1 < nq; q += 2) {
+          load(q + 1, a1, b1);
 #pragma unroll
-          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[t], bb[t], acc, 0, 0, 0);
+          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
+          if (q + 2 < nq) load(q + 2, a0, b0);
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            a[t] = an[t];
-            bb[t] = bn[t];
-          }
+          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[t], b1[t], acc, 0, 0, 0);
+        }
+        if (q < nq) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
         }
         const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
 #pragma unroll

@@ -75,7 +75,7 @@ class RenderRequest:
     jac: Optional[object] = None    # Jacobian_Image to accumulate columns into
     slot_cols: Optional[Dict[int, int]] = None  # slot -> Jacobian column
     dvdr: Optional[Dict[int, float]] = None     # slot -> d value / d rep
-    region: Optional[object] = None  # window the result is clipped to (default: image/jac window)
+    region: Optional[object] = None  # window the result is clipped to (default: destination & window)
 
 
 _UNIT_KERNEL = {}
@@ -295,7 +295,8 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     job = _new_job(pl, kh, kw)
     bx, by = pl.bx, pl.by
     if req.image is not None:
-        region = req.region if req.region is not None else req.image.window
+        # default: the part of the image the sampling window covers
+        region = req.region if req.region is not None else (req.image.window & sample_window)
         (dy, dx), (sy, sx), (h, w) = _box(req.image.window, sample_window, region)
         img = req.image.data
         if img.dtype != torch.float32 or not img.is_contiguous():
@@ -309,7 +310,9 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
             _launch("convolve", 2.0 * kh * kw * h * w, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j1), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
     if req.jac is not None and slot_cols:
-        region = req.region if req.region is not None else req.jac.window
+        # default: the part of the Jacobian the sampling window covers (a
+        # component on a sub-window of a group's Jacobian lands at its own pixels)
+        region = req.region if req.region is not None else (req.jac.window & sample_window)
         (dy, dx), (sy, sx), (h, w) = _box(req.jac.window, sample_window, region)
         jd = req.jac.data
         if not jd.is_contiguous() or jd.dtype != torch.float32:

@@ -83,6 +83,43 @@ def test_group_c0_like():
         assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k
 
 
+def test_group_subwindows_union_and_jacobian_scatter():
+    """Two Sersic components on overlapping sub-windows of a 72 x 64 image plus
+    a sky on all of it: the group renders over the union of the windows with
+    each component confined to its own window (group_model_object.py:92-129,
+    183-231), and each component's Jacobian columns land in the group
+    Jacobian at its own pixels, zero elsewhere (group_model_object.py:233-283,
+    jacobian_image.py:79-118) -- against the oracle."""
+    nx, ny = 72, 64
+    psf = H.gaussian_psf(1.2, 11, 1.0)
+    tgt = target(nx, ny, psf=psf)
+    wins = [((4, 44), (6, 50)), ((30, 70), (14, 60))]  # [xmin, xmax), [ymin, ymax) in pixels
+    pars = [{"center": [24.6, 27.2], "q": 0.6, "PA": 0.9, "n": 2.2, "Re": 5.0, "Ie": 1.1},
+            {"center": [49.3, 37.8], "q": 0.8, "PA": 2.1, "n": 1.1, "Re": 7.0, "Ie": 0.6}]
+    ocs, ms = [], []
+    for k, (((x0, x1), (y0, y1)), p) in enumerate(zip(wins, pars)):
+        g = H.Grid(x1 - x0, y1 - y0, ref_ij=(-0.5 - x0, -0.5 - y0))
+        ocs.append(H.Component("sersic", g, dict(p), psf=torch.as_tensor(psf, dtype=torch.float64), psf_mode="full"))
+        ms.append(ap.models.AstroPhot_Model(name=f"sub{k}", model_type="sersic galaxy model", target=tgt,
+                                            parameters=dict(p), psf_mode="full", window=[[x0, x1], [y0, y1]]))
+    osky, msky = component("flat_sky", {"F": math.log10(0.3)}, tgt, nx, ny)
+    om = H.Model(ocs + [osky], window=H.Grid(nx, ny))
+    grp = ap.models.AstroPhot_Model(name="subgrp", model_type="group model", models=ms + [msky], target=tgt)
+    rep = om.vector_representation()
+    assert np.allclose(rep.numpy(), grp.parameters.vector_representation().numpy())
+    assert rel_err(grp().data, om.forward(rep)) < IMG_TOL
+    J = grp.jacobian(parameters=rep.clone(), as_representation=True).data
+    Jref = om.jacobian(rep)
+    assert J.shape == Jref.shape
+    for k in range(J.shape[-1]):
+        assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k
+    # the first component's columns vanish outside its window
+    (x0, x1), (y0, y1) = wins[0]
+    outside = torch.ones(ny, nx, dtype=torch.bool)
+    outside[y0:y1, x0:x1] = False
+    assert float(J[..., :7].cpu()[outside].abs().max()) == 0.0
+
+
 def test_chi2_and_normal_equations():
     lib = ap._lib.load()
     torch.manual_seed(0)

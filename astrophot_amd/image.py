@@ -61,6 +61,11 @@ class Window:
 
     def __init__(self, *, pixel_shape=None, origin=None, center=None, pixelscale=None,
                  reference_imageij=(-0.5, -0.5), reference_imagexy=(0.0, 0.0), state=None, **kwargs):
+        # world-coordinate fields (image/wcs.py:36-43): carried through save/load;
+        # every computation here stays on the tangent plane
+        self.projection = kwargs.get("projection", "gnomonic")
+        self.reference_radec = _pair(kwargs.get("reference_radec", (0.0, 0.0)))
+        self.reference_planexy = _pair(kwargs.get("reference_planexy", (0.0, 0.0)))
         if state is not None:
             self.set_state(state)
             return
@@ -206,6 +211,8 @@ class Window:
         w._P, w._Pi, w._area = self._P, self._Pi, self._area
         w._nx, w._ny = self._nx, self._ny
         w._rij, w._rxy = self._rij, self._rxy
+        w.projection, w.reference_radec, w.reference_planexy = (self.projection, self.reference_radec,
+                                                                self.reference_planexy)
         if "pixelscale" in kwargs:
             w.pixelscale = kwargs["pixelscale"]
         if "pixel_shape" in kwargs:
@@ -328,10 +335,16 @@ class Window:
                 and self._p2x(0.0, 0.0) == other._p2x(0.0, 0.0))
 
     def get_state(self):
-        return {"pixel_shape": [self._nx, self._ny], "pixelscale": self.pixelscale.tolist(),
-                "reference_imageij": list(self._rij), "reference_imagexy": list(self._rxy)}
+        """The keys of image/wcs.py:361-370, 656-661 and window_object.py:311-314."""
+        return {"projection": self.projection, "reference_radec": list(self.reference_radec),
+                "reference_planexy": list(self.reference_planexy), "pixelscale": self.pixelscale.tolist(),
+                "reference_imageij": list(self._rij), "reference_imagexy": list(self._rxy),
+                "pixel_shape": [self._nx, self._ny]}
 
     def set_state(self, state):
+        self.projection = state.get("projection", "gnomonic")
+        self.reference_radec = _pair(state.get("reference_radec", (0.0, 0.0)))
+        self.reference_planexy = _pair(state.get("reference_planexy", (0.0, 0.0)))
         self.pixelscale = state.get("pixelscale", 1.0)
         self.pixel_shape = state["pixel_shape"]
         self._rij = _pair(state.get("reference_imageij", (-0.5, -0.5)))
@@ -348,7 +361,12 @@ class Image:
     north = math.pi / 2  # image/image_header.py:36
 
     def __init__(self, *, data=None, pixelscale=None, window: Optional[Window] = None, zeropoint=None,
-                 origin=None, center=None, identity=None, **kwargs):
+                 origin=None, center=None, identity=None, state=None, **kwargs):
+        if state is not None:  # image_object.py:397-399
+            header = state.get("header", {})
+            window = Window(state=header["window"])
+            zeropoint = header.get("zeropoint", zeropoint)
+            data = np.asarray(state["data"], dtype=np.float64)
         if data is None and window is None:
             raise InvalidData("Image must have either data or a window to construct itself.")
         if window is None:
@@ -415,6 +433,13 @@ class Image:
 
     def _new(self, data, window):
         return self.__class__(data=data, window=window, zeropoint=self.zeropoint, identity=self.identity)
+
+    def get_state(self):
+        """image_object.py:390-395 with the header keys of image_header.py:252-263."""
+        header = {"window": self.window.get_state()}
+        if self.zeropoint is not None:
+            header["zeropoint"] = float(self.zeropoint)
+        return {"type": self.__class__.__name__, "data": self.data.detach().cpu().tolist(), "header": header}
 
     def copy(self):
         return self._new(self.data.clone(), self.window.copy())

@@ -11,12 +11,15 @@ kernels (see ``_engine``); the class attributes are the reference's knobs
 
 from __future__ import annotations
 
+import io
+import json
 import math
 from collections import OrderedDict
 from typing import Optional, Sequence
 
 import numpy as np
 import torch
+import yaml
 
 from . import AP_config
 from . import _engine
@@ -44,7 +47,9 @@ class AstroPhot_Model:
     usable = False
     default_uncertainty = 1e-2
 
-    def __new__(cls, *, model_type=None, **kwargs):
+    def __new__(cls, *, model_type=None, filename=None, **kwargs):
+        if filename is not None:  # core_model.py:98-106: the saved model_type picks the class
+            model_type = AstroPhot_Model.load(filename)["model_type"]
         if model_type is not None:
             for M in _all_subclasses(AstroPhot_Model):
                 if M.model_type == model_type and M.usable:
@@ -56,6 +61,7 @@ class AstroPhot_Model:
         self.name = name if name is not None else f"{self.model_type}:{id(self)}"
         self.parameters = Parameter_Node(self.name)
         self._target = None
+        self._target_identity = None
         self._window = None
         self.target = target
         self.window = window
@@ -76,6 +82,8 @@ class AstroPhot_Model:
         if not (tar is None or isinstance(tar, Target_Image)):
             raise InvalidTarget("AstroPhot_Model target must be a Target_Image instance.")
         self._target = tar
+        if tar is not None:
+            self._target_identity = tar.identity
 
     @property
     def window(self):
@@ -127,6 +135,43 @@ class AstroPhot_Model:
     def initialize(self, target=None, parameters=None, **kwargs):
         pass
 
+    # -- state on disk (core_model.py:374-449) ------------------------------
+    def get_state(self, *args, **kwargs):
+        return {"name": self.name, "model_type": self.model_type}
+
+    def save(self, filename="AstroPhot.yaml"):
+        """Write ``get_state()`` as YAML (.yaml) or JSON (.json)."""
+        fname = str(filename)
+        if fname.endswith(".yaml"):
+            with open(fname, "w") as f:
+                yaml.safe_dump(self.get_state(), f, indent=2)
+        elif fname.endswith(".json"):
+            with open(fname, "w") as f:
+                json.dump(self.get_state(), f, indent=2)
+        elif fname.endswith(".hdf5"):
+            raise ValueError("hdf5 model files need h5py, which this installation lacks; use .yaml or .json")
+        else:
+            raise ValueError(f"Unrecognized filename format: {fname}, must be one of: .json, .yaml")
+
+    @classmethod
+    def load(cls, filename="AstroPhot.yaml"):
+        """The state dictionary held by ``filename``: a dict, a text stream of
+        YAML, or a .yaml / .json path.  YAML is read with the safe loader,
+        which builds plain dicts, lists, strings and numbers only."""
+        if isinstance(filename, dict):
+            return filename
+        if isinstance(filename, io.TextIOBase):
+            return yaml.safe_load(filename)
+        fname = str(filename)
+        if fname.endswith(".yaml"):
+            with open(fname, "r") as f:
+                return yaml.safe_load(f)
+        if fname.endswith(".json"):
+            with open(fname, "r") as f:
+                return json.load(f)
+        raise ValueError(f"Unrecognized filename format: {fname}, must be one of: .json, .yaml "
+                         "or a state dictionary")
+
 
 class Component_Model(AstroPhot_Model):
     """model_object.py:25-444."""
@@ -153,6 +198,9 @@ class Component_Model(AstroPhot_Model):
     def __init__(self, *, name=None, psf=None, parameters=None, **kwargs):
         super().__init__(name=name, **kwargs)
         self._psf = None
+        if "filename" in kwargs:  # model_object.py:122-125
+            self.load(kwargs["filename"], new_name=name)
+            return
         for key in self.track_attrs:
             if key in kwargs:
                 setattr(self, key, kwargs[key])
@@ -296,12 +344,45 @@ class Component_Model(AstroPhot_Model):
         out, col_index = _engine.vjp(self, self._values(), sample_window, G, G_window, slot_cols, dvdr)
         return out[[col_index[k] for k in range(n)]]
 
-    def get_state(self):
-        state = {"name": self.name, "model_type": self.model_type, "window": self.window.get_state(),
-                 "parameters": self.parameters.get_state()}
+    def get_state(self, save_params=True):
+        """model_object.py:408-428: window, parameters (unless a group saves
+        them), target identity, an image PSF, and the knobs that differ from
+        the class defaults."""
+        state = super().get_state()
+        state["window"] = self.window.get_state()
+        if save_params:
+            state["parameters"] = self.parameters.get_state()
+        state["target_identity"] = self._target_identity
+        if isinstance(self._psf, PSF_Image):
+            state["psf"] = self._psf.get_state()
         for key in self.track_attrs:
             if getattr(self, key) != getattr(self.__class__, key):
                 state[key] = getattr(self, key)
+        return state
+
+    def load(self, filename="AstroPhot.yaml", new_name=None):
+        """_model_methods.py:437-482: name, window, knobs, parameters and an
+        image PSF from a saved state.  The target is not saved; the one this
+        model holds stays."""
+        state = AstroPhot_Model.load(filename)
+        self.name = state["name"] if new_name is None else new_name
+        self.window = Window(**state["window"])
+        self._target_identity = state.get("target_identity")
+        self.target = self.target
+        for key in self.track_attrs:
+            if key in state:
+                setattr(self, key, state[key])
+        if isinstance(state["parameters"], Parameter_Node):
+            self.parameters = state["parameters"]
+        else:
+            self.parameters = Parameter_Node(self.name, state=state["parameters"])
+        missing = [p for p in self._parameter_order if p not in self.parameters.nodes]
+        if missing:
+            raise UnrecognizedModel(f"saved state of {self.name} lacks parameters {missing}")
+        if "psf" in state:
+            if state["psf"].get("type", "AstroPhot_Model") != "PSF_Image":
+                raise NotImplementedError("a saved PSF model is not supported; save the PSF as an image")
+            self.psf = PSF_Image(state=state["psf"])
         return state
 
 
@@ -379,6 +460,35 @@ class Sky_Model(Component_Model):
     def psf_mode(self):
         return "none"
 
+    def get_state(self, save_params=True):
+        """group_model_object.py:325-337: the group's parameter tree, and each
+        sub-model's state without its parameters (they sit in the tree)."""
+        state = super().get_state()
+        if save_params:
+            state["parameters"] = self.parameters.get_state()
+        state["models"] = {m.name: m.get_state(save_params=False) for m in self.models.values()}
+        return state
+
+    def load(self, filename="AstroPhot.yaml", new_name=None):
+        """group_model_object.py:339-365: rebuild the parameter tree, then load
+        each sub-model onto its branch (sub-models this group lacks are made
+        from their saved model_type)."""
+        state = AstroPhot_Model.load(filename)
+        self.name = state["name"] if new_name is None else new_name
+        if isinstance(state["parameters"], Parameter_Node):
+            self.parameters = state["parameters"]
+        else:
+            self.parameters = Parameter_Node(self.name, state=state["parameters"])
+        for mname, sub in state["models"].items():
+            sub = dict(sub)
+            sub["parameters"] = self.parameters[mname]
+            if mname in self.models:
+                self.models[mname].load(sub)
+            else:
+                self.add_model(AstroPhot_Model(name=mname, filename=sub, target=self.target))
+        self.update_window()
+        return state
+
     @psf_mode.setter
     def psf_mode(self, val):
         pass
@@ -420,6 +530,8 @@ class Group_Model(AstroPhot_Model):
         if models is not None:
             self.add_model(models)
         self.update_window()
+        if "filename" in kwargs:  # group_model_object.py:64-65
+            self.load(kwargs["filename"], new_name=name)
 
     def add_model(self, model):
         if isinstance(model, (tuple, list)):
@@ -446,6 +558,35 @@ class Group_Model(AstroPhot_Model):
     @property
     def psf_mode(self):
         return "none"
+
+    def get_state(self, save_params=True):
+        """group_model_object.py:325-337: the group's parameter tree, and each
+        sub-model's state without its parameters (they sit in the tree)."""
+        state = super().get_state()
+        if save_params:
+            state["parameters"] = self.parameters.get_state()
+        state["models"] = {m.name: m.get_state(save_params=False) for m in self.models.values()}
+        return state
+
+    def load(self, filename="AstroPhot.yaml", new_name=None):
+        """group_model_object.py:339-365: rebuild the parameter tree, then load
+        each sub-model onto its branch (sub-models this group lacks are made
+        from their saved model_type)."""
+        state = AstroPhot_Model.load(filename)
+        self.name = state["name"] if new_name is None else new_name
+        if isinstance(state["parameters"], Parameter_Node):
+            self.parameters = state["parameters"]
+        else:
+            self.parameters = Parameter_Node(self.name, state=state["parameters"])
+        for mname, sub in state["models"].items():
+            sub = dict(sub)
+            sub["parameters"] = self.parameters[mname]
+            if mname in self.models:
+                self.models[mname].load(sub)
+            else:
+                self.add_model(AstroPhot_Model(name=mname, filename=sub, target=self.target))
+        self.update_window()
+        return state
 
     def sample(self, image=None, window=None, parameters=None):
         """group_model_object.py:183-231.

@@ -72,8 +72,9 @@ class Parameter_Node:
     global_unlock = False
 
     def __init__(self, name, value=None, limits=(None, None), cyclic=False, locked=False,
-                 units="none", uncertainty=None, shape=None, **kwargs):
+                 units="none", uncertainty=None, shape=None, state=None, **kwargs):
         self.name = name
+        self._identity = None
         self.nodes: "OrderedDict[str, Parameter_Node]" = OrderedDict()
         self.units = units
         self.cyclic = bool(cyclic)
@@ -84,6 +85,9 @@ class Parameter_Node:
         self._uncertainty = None
         self._mask = None
         self.shape = shape
+        if state is not None:
+            self.set_state(state)
+            return
         if value is not None:
             self.value = value
         if uncertainty is not None:
@@ -96,7 +100,8 @@ class Parameter_Node:
 
     @property
     def identity(self):
-        return id(self)
+        """param/base.py:124-132: the identity a saved state carried, else id()."""
+        return id(self) if self._identity is None else self._identity
 
     def link(self, node):
         self.nodes[node.name] = node
@@ -322,24 +327,51 @@ class Parameter_Node:
 
     # -- state --------------------------------------------------------------
     def get_state(self):
-        """param/parameter.py:598-623."""
-        state = {"name": self.name}
-        if self.leaf:
-            if self._value is not None:
-                state["value"] = self._value.tolist()
-            if self._uncertainty is not None:
-                state["uncertainty"] = self._uncertainty.tolist()
-            if self.units != "none":
-                state["units"] = self.units
-            if self.limits != (None, None):
-                state["limits"] = list(self.limits)
-            if self.cyclic:
-                state["cyclic"] = True
-        else:
-            state["nodes"] = [n.get_state() for n in self.nodes.values()]
+        """The keys of param/base.py:134-148 and param/parameter.py:598-630:
+        name, identity, locked (when locked), nodes (a group's children), and
+        for a leaf value, shape, units, uncertainty, limits (None for an open
+        side) and cyclic (when cyclic)."""
+        state = {"name": self.name, "identity": self.identity}
         if self.locked:
             state["locked"] = True
+        if not self.leaf:
+            state["nodes"] = [n.get_state() for n in self.nodes.values()]
+        if self._value is not None:
+            state["value"] = self._value.detach().cpu().tolist()
+        if self.shape is not None and self._value is not None:
+            state["shape"] = list(self.shape)
+        if self.units is not None:
+            state["units"] = self.units
+        if self._uncertainty is not None:
+            state["uncertainty"] = self._uncertainty.detach().cpu().tolist()
+        if self.limits != (None, None):
+            state["limits"] = list(self.limits)
+        if self.cyclic:
+            state["cyclic"] = True
         return state
+
+    def set_state(self, state):
+        """param/base.py:150-163 and param/parameter.py:632-645: rebuild this
+        node, and its children, from a ``get_state`` dictionary."""
+        self.name = state["name"]
+        self._identity = state.get("identity")
+        self.nodes = OrderedDict()
+        for sub in state.get("nodes", []):
+            self.link(Parameter_Node(sub["name"], state=sub))
+        self.units = state.get("units", "none")
+        lim = state.get("limits") or (None, None)
+        self.limits = (None if lim[0] is None else float(lim[0]), None if lim[1] is None else float(lim[1]))
+        self.cyclic = bool(state.get("cyclic", False))
+        self.locked = False
+        self._value = self._uncertainty = self._mask = None
+        self.shape = tuple(state["shape"]) if state.get("shape") is not None else None
+        if state.get("value") is not None:
+            if isinstance(state["value"], str):
+                raise InvalidParameter(f"{self.name}: linked or functional values ({state['value']}) are not supported")
+            self._set_val_self(state["value"])
+        if state.get("uncertainty") is not None:
+            self.uncertainty = state["uncertainty"]
+        self.locked = bool(state.get("locked", False))
 
     def __repr__(self):
         if self.leaf:

@@ -120,6 +120,26 @@ def test_group_subwindows_union_and_jacobian_scatter():
     assert float(J[..., :7].cpu()[outside].abs().max()) == 0.0
 
 
+def test_saved_group_renders_the_same(tmp_path):
+    """A group saved to YAML and loaded by filename (core_model.py:386-449,
+    group_model_object.py:325-365) renders, and differentiates, bit for bit
+    as the group it was saved from."""
+    nx, ny = 60, 52
+    tgt = target(nx, ny, psf=H.gaussian_psf(1.3, 11, 1.0))
+    a = ap.models.AstroPhot_Model(name="a", model_type="sersic galaxy model", target=tgt, psf_mode="full",
+                                  parameters={"center": [24.3, 22.8], "q": 0.7, "PA": 0.4, "n": 1.7,
+                                              "Re": 6.0, "Ie": 0.9}, window=[[2, 50], [0, 48]])
+    b = ap.models.AstroPhot_Model(name="b", model_type="moffat galaxy model", target=tgt,
+                                  parameters={"center": [40.1, 30.6], "q": 0.8, "PA": 2.0, "n": 2.5,
+                                              "Rd": 3.0, "I0": 0.5})
+    grp = ap.models.AstroPhot_Model(name="g", model_type="group model", models=[a, b], target=tgt)
+    fn = str(tmp_path / "g.yaml")
+    grp.save(fn)
+    new = ap.models.AstroPhot_Model(name="g", filename=fn, target=tgt)
+    assert torch.equal(new().data, grp().data)
+    assert torch.equal(new.jacobian().data, grp.jacobian().data)
+
+
 def test_chi2_and_normal_equations():
     lib = ap._lib.load()
     torch.manual_seed(0)

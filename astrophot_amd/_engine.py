@@ -45,6 +45,9 @@ PROFILE_SLOTS = {
     "flat_sky": {"F": L.APX_P_A},
 }
 GEOM_SLOTS = {"q": L.APX_P_Q, "PA": L.APX_P_PA}
+# parameters of an auxiliary PSF model (model_object.py:133-147) are keyed
+# PSF_SLOT0 + their slot in the PSF model
+PSF_SLOT0 = 16
 
 
 def gauss_legendre(n: int):
@@ -159,6 +162,13 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Pl
     sm = model.sampling_mode
     if sm == "midpoint":
         cp.sampling = L.APX_SAMPLE_MIDPOINT
+    elif sm == "simpsons":
+        # _model_methods.py:97-109: corners, edge midpoints and centre of each
+        # pixel weighted 1, 4, 16 / 36, the centre sample as the reference
+        cp.sampling = L.APX_SAMPLE_QUAD
+        cp.sample_quad = 3
+        cp.sq_node[:3] = (-1.0, 0.0, 1.0)
+        cp.sq_weight[:3] = (1.0 / 3.0, 4.0 / 3.0, 1.0 / 3.0)
     elif sm.startswith("quad"):
         cp.sampling = L.APX_SAMPLE_QUAD
         cp.sample_quad = int(sm[sm.find(":") + 1:])
@@ -168,7 +178,7 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Pl
         cp.sq_node[:cp.sample_quad], cp.sq_weight[:cp.sample_quad] = x, w
     else:
         raise SpecificationConflict(
-            f"{model.name} has unknown sampling mode: {sm}. Should be one of: midpoint, quad:level")
+            f"{model.name} has unknown sampling mode: {sm}. Should be one of: midpoint, simpsons, quad:level")
     im = model.integrate_mode if kind != "flat_sky" else "none"
     if im not in ("none", "threshold"):
         raise SpecificationConflict(
@@ -216,6 +226,11 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Pl
     cp.tolerance = float(model.sampling_tolerance)
     if kind == "flat_sky":
         cp.q, cp.pa = 1.0, 0.0
+    elif getattr(model, "_is_psf", False):
+        # PSF models evaluate on untransformed coordinates (psf_model_object.py:
+        # 136-137): q = 1 and a position angle on the reference axis make the
+        # kernel's rotation the identity
+        cp.q, cp.pa = 1.0, math.pi / 2
     else:
         cp.q, cp.pa = float(values["q"]), float(values["PA"])
     for name, slot in PROFILE_SLOTS[kind].items():
@@ -236,32 +251,94 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Pl
         for c, s in enumerate(chan_slots):
             cp.chan_slot[c + 1] = s
     for s, v in (dvdr or {}).items():
-        cp.dvdr[s] = float(v)
+        if s < L.APX_NSLOT:  # an auxiliary PSF model's slots are not this render's
+            cp.dvdr[s] = float(v)
     return Plan(cp, window, bx, by, shift, psf_full, chan_slots)
 
 
-def _kernels(model, pl, dvdr, dev, st):
-    """Shifted, normalised, flipped PSF (+ centre derivatives) or a unit kernel:
-    (device pointer, kh, kw, tensor view of the taps)."""
+@dataclass
+class Kernels:
+    """Convolution kernels of one render: ``ptr`` -> [K] or [K, dK/dcx, dK/dcy]
+    (kh x kw each, flipped); ``dk_ptr`` -> [K, dK_0 .. dK_(nd-1), 0] for the
+    parameters ``dk_slots`` of an auxiliary PSF model."""
+
+    ptr: int
+    kh: int
+    kw: int
+    view: torch.Tensor
+    dk_ptr: Optional[int] = None
+    dk_slots: Tuple[int, ...] = ()
+
+
+def psf_render(psf_model, dev, st, slots=(), dvdr=None):
+    """Sample a PSF model on its window and normalise it (psf_model_object.py:
+    180-264): (planes (1 + len(slots), ph, pw) float32 view, plane slots)."""
     lib = L.load()
-    if pl.psf_full:
+    pl = plan(psf_model, psf_model._values(), psf_model.window, {s: 0 for s in slots}, dvdr or {},
+              channels=bool(slots))
+    cp = pl.cp
+    n = cp.nx * cp.ny
+    planes = _buffer(psf_model, "planes", 4 * cp.nchan * n, dev)
+    wsb = lib.apx_render_workspace_bytes(n)
+    ws = _buffer(psf_model, "render_ws", wsb, dev)
+    L.check(lib.apx_render_component(ctypes.byref(cp), L.ptr(planes), L.ptr(ws), wsb, st))
+    if psf_model.normalize_psf:
+        L.check(lib.apx_normalize_planes(L.ptr(planes), cp.nchan, n, st))
+    return planes[:4 * cp.nchan * n].view(torch.float32).view(cp.nchan, cp.ny, cp.nx), pl.chan_slots
+
+
+def _kernels(model, pl, dvdr, dev, st, psf_slots=()) -> Kernels:
+    """Shifted, normalised, flipped PSF (+ centre derivatives, + derivatives
+    in the parameters of an auxiliary PSF model) or a unit kernel."""
+    lib = L.load()
+    if not pl.psf_full:
+        k = _unit_kernel(dev)
+        return Kernels(L.ptr(k), 1, 1, k)
+    aux = model._aux_psf()
+    dplanes = None
+    psf_slots = tuple(psf_slots) if aux is not None else ()
+    if aux is not None:
+        pdvdr = {s - PSF_SLOT0: d for s, d in dvdr.items() if s >= PSF_SLOT0}
+        planes, chan = psf_render(aux, dev, st, psf_slots, pdvdr)
+        psf_slots = tuple(chan)
+        pdata = planes[0]
+        dplanes = planes[1:] if psf_slots else None
+    else:
         pdata = model.psf.data
         if pdata.dtype != torch.float32 or not pdata.is_contiguous():
             pdata = pdata.to(torch.float32).contiguous()
-        ph, pw = int(pdata.shape[0]), int(pdata.shape[1])
-        if pl.shift is not None:
-            kbuf = _buffer(model, "kernels", 4 * 3 * (ph + 2) * (pw + 2), dev)
-            pinv = (ctypes.c_double * 4)(*pl.window._Pi)
-            L.check(lib.apx_psf_shift(L.ptr(pdata), ph, pw, pl.shift[0], pl.shift[1], pinv,
-                                      float(dvdr.get(L.APX_P_CX, 1.0)), float(dvdr.get(L.APX_P_CY, 1.0)),
-                                      L.ptr(kbuf), st))
-            view = kbuf[:4 * 3 * (ph + 2) * (pw + 2)].view(torch.float32).view(3, ph + 2, pw + 2)
-            return L.ptr(kbuf), ph + 2, pw + 2, view
-        kernels = torch.flip(pdata / pdata.sum(), dims=(0, 1)).reshape(1, ph, pw).contiguous()
-        model.__dict__.setdefault("_apx_buffers", {})["kernels_t"] = kernels
-        return L.ptr(kernels), ph, pw, kernels
-    k = _unit_kernel(dev)
-    return L.ptr(k), 1, 1, k
+    ph, pw = int(pdata.shape[0]), int(pdata.shape[1])
+    shifted = pl.shift is not None
+    kh, kw = (ph + 2, pw + 2) if shifted else (ph, pw)
+    sx, sy = pl.shift if shifted else (0.0, 0.0)
+    dk_ptr = None
+    if not shifted or psf_slots:
+        nd = len(psf_slots)
+        dbuf = _buffer(model, "dkernels", 4 * (nd + 2) * kh * kw, dev)
+        L.check(lib.apx_psf_kernels_dparam(L.ptr(pdata), L.ptr(dplanes) if nd else 0, nd, ph, pw,
+                                           1 if shifted else 0, sx, sy, L.ptr(dbuf), st))
+        dk_ptr = L.ptr(dbuf)
+    if shifted:
+        kbuf = _buffer(model, "kernels", 4 * 3 * kh * kw, dev)
+        pinv = (ctypes.c_double * 4)(*pl.window._Pi)
+        L.check(lib.apx_psf_shift(L.ptr(pdata), ph, pw, sx, sy, pinv,
+                                  float(dvdr.get(L.APX_P_CX, 1.0)), float(dvdr.get(L.APX_P_CY, 1.0)),
+                                  L.ptr(kbuf), st))
+        view = kbuf[:4 * 3 * kh * kw].view(torch.float32).view(3, kh, kw)
+        return Kernels(L.ptr(kbuf), kh, kw, view, dk_ptr, psf_slots)
+    view = dbuf[:4 * kh * kw].view(torch.float32).view(1, kh, kw)
+    return Kernels(dk_ptr, kh, kw, view, dk_ptr, psf_slots)
+
+
+def _dk_pairs(kern: Kernels, psf_cols):
+    """(kernel pointer, (column, column)) per pair of auxiliary-PSF columns."""
+    plane = 4 * kern.kh * kern.kw
+    out = []
+    for k in range(0, len(kern.dk_slots), 2):
+        c0 = psf_cols[kern.dk_slots[k]]
+        c1 = psf_cols[kern.dk_slots[k + 1]] if k + 1 < len(kern.dk_slots) else -1
+        out.append((kern.dk_ptr + k * plane, (c0, c1)))
+    return out
 
 
 def _new_job(pl, kh, kw):
@@ -277,7 +354,9 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     ``req.window``; accumulate into ``req.image`` / ``req.jac``."""
     lib = L.load()
     dvdr = req.dvdr or {}
-    slot_cols = req.slot_cols or {}
+    all_cols = req.slot_cols or {}
+    slot_cols = {s: c for s, c in all_cols.items() if s < PSF_SLOT0}
+    psf_cols = {s - PSF_SLOT0: c for s, c in all_cols.items() if s >= PSF_SLOT0}
     pl = plan(model, values, req.window, slot_cols, dvdr, channels=req.jac is not None)
     cp, nx, ny = pl.cp, pl.cp.nx, pl.cp.ny
     dev = AP_config.ap_device
@@ -287,7 +366,10 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     st = L.stream()
     _launch("render", nx * ny, lambda: L.check(
         lib.apx_render_component(ctypes.byref(cp), L.ptr(planes), L.ptr(ws), wsb, st)))
-    kptr, kh, kw, _ = _kernels(model, pl, dvdr, dev, st)
+    if getattr(model, "_is_psf", False) and model.normalize_psf:
+        L.check(lib.apx_normalize_planes(L.ptr(planes), cp.nchan, nx * ny, st))
+    kern = _kernels(model, pl, dvdr, dev, st, sorted(psf_cols) if req.jac is not None else ())
+    kptr, kh, kw = kern.ptr, kern.kh, kern.kw
     # crop back to the sampling window, then window-aware accumulation
     sample_window = req.window
     cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
@@ -309,7 +391,7 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
         if h > 0 and w > 0:
             _launch("convolve", 2.0 * kh * kw * h * w, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j1), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
-    if req.jac is not None and slot_cols:
+    if req.jac is not None and (slot_cols or psf_cols):
         # default: the part of the Jacobian the sampling window covers (a
         # component on a sub-window of a group's Jacobian lands at its own pixels)
         region = req.region if req.region is not None else (req.jac.window & sample_window)
@@ -327,10 +409,18 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
         if pl.psf_full and pl.shift is not None:
             j2.dk_col[0] = slot_cols.get(L.APX_P_CX, -1)
             j2.dk_col[1] = slot_cols.get(L.APX_P_CY, -1)
-        if h > 0 and w > 0:
-            nk = (cp.nchan - 1) + (2 if j2.dk_col[0] >= 0 or j2.dk_col[1] >= 0 else 0)
+        nk = (cp.nchan - 1) + (2 if j2.dk_col[0] >= 0 or j2.dk_col[1] >= 0 else 0)
+        if h > 0 and w > 0 and nk > 0:
             _launch("convolve_jac", 2.0 * kh * kw * h * w * nk, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j2), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
+        # auxiliary PSF parameters: conv(value plane, dK/dtheta), two columns a launch
+        for ptr, (c0, c1) in (_dk_pairs(kern, psf_cols) if h > 0 and w > 0 else ()):
+            j3 = _job_copy(j2)
+            j3.nchan = 1
+            j3.chan_col[:] = [-1] * L.APX_MAX_CHAN
+            j3.dk_col[0], j3.dk_col[1] = c0, c1
+            _launch("convolve_jac", 2.0 * kh * kw * h * w * 2, lambda: L.check(
+                lib.apx_convolve(ctypes.byref(j3), L.ptr(planes), ptr, L.ptr(cws), cwsb, st)))
     return ws
 
 
@@ -343,26 +433,31 @@ def vjp(model, values, window, G, G_window, slot_cols, dvdr, region=None):
     G (float32, G_window's shape) is carried back through the convolution
     (correlation with the unflipped shifted PSF), then apx_render_vjp
     recomputes the render and reduces against it; the centre columns of a
-    shifted-PSF model are sum(G * conv(deep, dK/dc))."""
+    shifted-PSF model, and the parameters of an auxiliary PSF model, are
+    sum(G * conv(deep, dK)) with dK the kernel's derivative."""
     lib = L.load()
     dev = AP_config.ap_device
     st = L.stream()
+    all_cols = slot_cols
+    slot_cols = {s: c for s, c in all_cols.items() if s < PSF_SLOT0}
+    psf_cols = {s - PSF_SLOT0: c for s, c in all_cols.items() if s >= PSF_SLOT0}
     pl = plan(model, values, window, slot_cols, dvdr, channels=True)
     cp, nx, ny, bx, by = pl.cp, pl.cp.nx, pl.cp.ny, pl.bx, pl.by
     region = G_window if region is None else region
     (dy, dx), (sy, sx), (h, w) = _box(G_window, window, region)
-    out = torch.zeros(len(slot_cols), dtype=torch.float64, device=dev)
-    col_index = {c: k for k, c in enumerate(sorted(slot_cols.values()))}
+    out = torch.zeros(len(all_cols), dtype=torch.float64, device=dev)
+    col_index = {c: k for k, c in enumerate(sorted(all_cols.values()))}
     if h <= 0 or w <= 0:
         return out, col_index
     Gw = torch.zeros((ny, nx), dtype=torch.float32, device=dev)
     Gw[by + sy:by + sy + h, bx + sx:bx + sx + w] = G[dy:dy + h, dx:dx + w]
-    kptr, kh, kw, kview = _kernels(model, pl, dvdr, dev, st)
+    kern = _kernels(model, pl, dvdr, dev, st, sorted(psf_cols))
+    kh, kw = kern.kh, kern.kw
     cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
     cws = _buffer(model, "conv_ws", cwsb, dev)
     if pl.psf_full:
         # B = adjoint of the convolution: correlation with the unflipped taps
-        taps = torch.flip(kview[0], dims=(0, 1)).contiguous()
+        taps = torch.flip(kern.view[0], dims=(0, 1)).contiguous()
         B = torch.zeros((ny, nx), dtype=torch.float32, device=dev)
         jb = _new_job(pl, kh, kw)
         jb.nchan = 1
@@ -375,25 +470,34 @@ def vjp(model, values, window, G, G_window, slot_cols, dvdr, region=None):
     value = _buffer(model, "planes", 4 * ny * nx, dev)
     wsb = lib.apx_render_workspace_bytes(nx * ny)
     ws = _buffer(model, "render_ws", wsb, dev)
+    centre_dk = pl.psf_full and pl.shift is not None and (L.APX_P_CX in slot_cols or L.APX_P_CY in slot_cols)
+    psf_dk = pl.psf_full and bool(kern.dk_slots) and bool(psf_cols)
     if cp.nchan > 1:
         g = torch.empty(cp.nchan - 1, dtype=torch.float64, device=dev)
         L.check(lib.apx_render_vjp(ctypes.byref(cp), L.ptr(B), L.ptr(value), L.ptr(g), L.ptr(ws), wsb, st))
         for c, s in enumerate(pl.chan_slots):
             out[col_index[slot_cols[s]]] += g[c]
-    elif pl.psf_full and pl.shift is not None:
+    elif centre_dk or psf_dk:
         L.check(lib.apx_render_component(ctypes.byref(cp), L.ptr(value), L.ptr(ws), wsb, st))
-    if pl.psf_full and pl.shift is not None and (L.APX_P_CX in slot_cols or L.APX_P_CY in slot_cols):
-        C = torch.zeros((h, w, 2), dtype=torch.float32, device=dev)
-        jc = _new_job(pl, kh, kw)
-        jc.nchan = 1
-        jc.x0, jc.y0, jc.ow, jc.oh = bx + sx, by + sy, w, h
-        jc.jac, jc.jac_row, jc.jac_w, jc.jac_h, jc.jac_np, jc.jac_x0, jc.jac_y0 = L.ptr(C), 2 * w, w, h, 2, 0, 0
-        jc.dk_col[0], jc.dk_col[1] = 0, 1
-        L.check(lib.apx_convolve(ctypes.byref(jc), L.ptr(value), kptr, L.ptr(cws), cwsb, st))
+    pairs = []
+    if centre_dk:
+        pairs.append((kern.ptr, (slot_cols.get(L.APX_P_CX, -1), slot_cols.get(L.APX_P_CY, -1))))
+    if psf_dk:
+        pairs.extend(_dk_pairs(kern, psf_cols))
+    if pairs:
         Gb = G[dy:dy + h, dx:dx + w].double()
-        for k, s in enumerate((L.APX_P_CX, L.APX_P_CY)):
-            if s in slot_cols:
-                out[col_index[slot_cols[s]]] += (Gb * C[..., k].double()).sum()
+        C = torch.empty((h, w, 2), dtype=torch.float32, device=dev)
+        for ptr, cols in pairs:
+            C.zero_()
+            jc = _new_job(pl, kh, kw)
+            jc.nchan = 1
+            jc.x0, jc.y0, jc.ow, jc.oh = bx + sx, by + sy, w, h
+            jc.jac, jc.jac_row, jc.jac_w, jc.jac_h, jc.jac_np, jc.jac_x0, jc.jac_y0 = L.ptr(C), 2 * w, w, h, 2, 0, 0
+            jc.dk_col[0], jc.dk_col[1] = (0 if cols[0] >= 0 else -1), (1 if cols[1] >= 0 else -1)
+            L.check(lib.apx_convolve(ctypes.byref(jc), L.ptr(value), ptr, L.ptr(cws), cwsb, st))
+            for k in range(2):
+                if cols[k] >= 0:
+                    out[col_index[cols[k]]] += (Gb * C[..., k].double()).sum()
     return out, col_index
 
 

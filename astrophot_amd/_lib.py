@@ -67,6 +67,8 @@ SIGNATURES = {
     "apx_render_vjp": (ctypes.c_int, [ctypes.POINTER(Component), _p, _p, _p, _p, _i64, _p]),
     "apx_render_selected_count": (ctypes.c_int, [_p, _p, ctypes.POINTER(_i32)]),
     "apx_psf_shift": (ctypes.c_int, [_p, _i32, _i32, _f64, _f64, _p, _f64, _f64, _p, _p]),
+    "apx_normalize_planes": (ctypes.c_int, [_p, _i32, _i64, _p]),
+    "apx_psf_kernels_dparam": (ctypes.c_int, [_p, _p, _i32, _i32, _i32, _i32, _f64, _f64, _p, _p]),
     "apx_convolve_workspace_bytes": (_i64, [_i32, _i32]),
     "apx_convolve": (ctypes.c_int, [ctypes.POINTER(ConvJob), _p, _p, _p, _i64, _p]),
     "apx_reduce_workspace_bytes": (_i64, [_i64, _i32]),

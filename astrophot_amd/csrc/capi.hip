@@ -58,6 +58,17 @@ int apx_psf_shift(const float* psf, int32_t ph, int32_t pw, double shift_x, doub
                         static_cast<hipStream_t>(stream));
 }
 
+int apx_normalize_planes(float* planes, int32_t nchan, int64_t npix, void* stream) {
+  return apx::normalize_planes(planes, nchan, npix, static_cast<hipStream_t>(stream));
+}
+
+int apx_psf_kernels_dparam(const float* psf, const float* dpsf, int32_t nd, int32_t ph, int32_t pw,
+                           int32_t shifted, double shift_x, double shift_y, float* out,
+                           void* stream) {
+  return apx::psf_kernels_dparam(psf, dpsf, nd, ph, pw, shifted, shift_x, shift_y, out,
+                                 static_cast<hipStream_t>(stream));
+}
+
 int64_t apx_convolve_workspace_bytes(int32_t kh, int32_t kw) {
   return apx::convolve_workspace_bytes(kh, kw);
 }

@@ -21,9 +21,12 @@ int render_component(const apx_component* cp, float* planes, void* workspace, in
 int render_vjp(const apx_component* cp, const float* weights, float* value_plane, double* grad,
                void* workspace, int64_t ws_bytes, hipStream_t stream);
 int render_selected_count(const void* workspace, hipStream_t stream, int32_t* count);
+int normalize_planes(float* planes, int32_t nchan, int64_t npix, hipStream_t stream);
 
 int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
               double dvdr_x, double dvdr_y, float* kernels, hipStream_t stream);
+int psf_kernels_dparam(const float* psf, const float* dpsf, int32_t nd, int32_t ph, int32_t pw,
+                       int32_t shifted, double sx, double sy, float* out, hipStream_t stream);
 int64_t convolve_workspace_bytes(int32_t kh, int32_t kw);
 int convolve(const apx_conv_job* job, const float* planes, const float* kernels, void* workspace,
              int64_t ws_bytes, hipStream_t stream);

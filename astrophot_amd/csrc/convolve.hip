@@ -73,6 +73,48 @@ __global__ __launch_bounds__(256) void psf_shift_kernel(const float* psf, int ph
   }
 }
 
+// Kernels of a PSF that is itself rendered from parameters theta (an
+// auxiliary PSF_Model, model_object.py:305-310).  With T the PSF after the
+// sub-pixel shift (the padded bilinear shift above, or the PSF itself when
+// unshifted) and K = T / sum T (_model_methods.py:243),
+//   dK/dtheta = dT / sum T - T (sum dT) / (sum T)^2 ,
+// dT being the same shift applied to the derivative plane dP/dtheta.
+// Block 0 writes K, block d + 1 writes dK of derivative plane d, and block
+// nd + 1 writes a zero plane; every plane flipped for correlation.
+__global__ __launch_bounds__(256) void psf_kernels_dparam_kernel(const float* psf, const float* dpsf,
+                                                                 int nd, int ph, int pw, int shifted,
+                                                                 double sx, double sy, float* out) {
+  __shared__ double red[4];
+  const int H = shifted ? ph + 2 : ph, W = shifted ? pw + 2 : pw, n = H * W;
+  const int b = blockIdx.x;
+  float* o = out + (int64_t)b * n;
+  if (b == nd + 1) {
+    for (int k = threadIdx.x; k < n; k += 256) o[k] = 0.0f;
+    return;
+  }
+  const float* dp = b > 0 ? dpsf + (int64_t)(b - 1) * ph * pw : nullptr;
+  auto sample = [&](const float* src, int r, int c) -> double {
+    if (!shifted) return (double)src[r * pw + c];
+    double v, dx, dy;
+    bilinear(src, ph, pw, r, c, sx, sy, v, dx, dy);
+    return v;
+  };
+  double S = 0.0, Sd = 0.0;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    S += sample(psf, k / W, k % W);
+    if (dp) Sd += sample(dp, k / W, k % W);
+  }
+  S = block_sum256(S, red);
+  Sd = block_sum256(Sd, red);
+  const double iS = 1.0 / S;
+  for (int k = threadIdx.x; k < n; k += 256) {
+    const int r = k / W, c = k % W;
+    const double t = sample(psf, r, c);
+    const double v = dp ? sample(dp, r, c) * iS - t * Sd * iS * iS : t * iS;
+    o[(H - 1 - r) * W + (W - 1 - c)] = (float)v;
+  }
+}
+
 // One workgroup = kTY x kTX outputs for every render plane.
 template <bool DK>
 __global__ __launch_bounds__(256) void convolve_kernel(apx_conv_job job, const float* __restrict__ planes,
@@ -248,6 +290,20 @@ int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, co
   hipLaunchKernelGGL(psf_shift_kernel, dim3(1), dim3(256), 0, stream, psf, ph, pw, sx, sy, pinv[0],
                      pinv[1], pinv[2], pinv[3], dvdr_x, dvdr_y, kernels);
   return check_launch("psf_shift");
+}
+
+int psf_kernels_dparam(const float* psf, const float* dpsf, int32_t nd, int32_t ph, int32_t pw,
+                       int32_t shifted, double sx, double sy, float* out, hipStream_t stream) {
+  if (!psf || !out || (nd > 0 && !dpsf))
+    return set_error(APX_E_INVALID, "psf_kernels_dparam: null pointer");
+  if (nd < 0 || nd > APX_NSLOT) return set_error(APX_E_INVALID, "psf_kernels_dparam: nd out of range");
+  if (ph < 1 || pw < 1 || (ph % 2) == 0 || (pw % 2) == 0)
+    return set_error(APX_E_INVALID, "psf_kernels_dparam: psf must have odd shape");
+  if (ph + 2 > APX_MAX_KERNEL || pw + 2 > APX_MAX_KERNEL)
+    return set_error(APX_E_LIMIT, "psf_kernels_dparam: psf larger than APX_MAX_KERNEL - 2");
+  hipLaunchKernelGGL(psf_kernels_dparam_kernel, dim3((unsigned)(nd + 2)), dim3(256), 0, stream, psf,
+                     dpsf, nd, ph, pw, shifted ? 1 : 0, sx, sy, out);
+  return check_launch("psf_kernels_dparam");
 }
 
 int64_t convolve_workspace_bytes(int32_t kh, int32_t kw) {

@@ -297,7 +297,56 @@ __global__ __launch_bounds__(64) void render_vjp_final_kernel(apx_component cp, 
 
 __global__ void zero_count_kernel(RenderWs ws) { ws.count[0] = 0; }
 
+// PSF_Model.sample's normalisation to unit sum (psf_model_object.py:255-257)
+// applied to render planes: plane 0 becomes P / S with S = sum P, and each
+// derivative plane dP becomes the derivative of P / S, dP / S - P (sum dP) / S^2.
+// One workgroup of 16 wavefronts; float64 sums in a fixed order.
+__global__ __launch_bounds__(1024) void normalize_planes_kernel(float* planes, int nchan,
+                                                                int64_t npix) {
+  __shared__ double red[APX_MAX_CHAN][16];
+  __shared__ double tot[APX_MAX_CHAN];
+  double s[APX_MAX_CHAN];
+#pragma unroll
+  for (int c = 0; c < APX_MAX_CHAN; ++c) s[c] = 0.0;
+  for (int64_t p = threadIdx.x; p < npix; p += 1024) {
+#pragma unroll
+    for (int c = 0; c < APX_MAX_CHAN; ++c)
+      if (c < nchan) s[c] += (double)planes[c * npix + p];
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < APX_MAX_CHAN; ++c) {
+    if (c < nchan) {
+      const double v = wave_sum(s[c]);
+      if (lane == 0) red[c][w] = v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nchan) {
+    double t = 0.0;
+    for (int k = 0; k < 16; ++k) t += red[threadIdx.x][k];
+    tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+  const double iS = 1.0 / tot[0];
+  for (int64_t p = threadIdx.x; p < npix; p += 1024) {
+    const double v = (double)planes[p];
+    for (int c = 1; c < nchan; ++c)
+      planes[c * npix + p] = (float)((double)planes[c * npix + p] * iS - v * tot[c] * iS * iS);
+    planes[p] = (float)(v * iS);
+  }
+}
+
 }  // namespace
+
+int normalize_planes(float* planes, int32_t nchan, int64_t npix, hipStream_t stream) {
+  if (!planes) return set_error(APX_E_INVALID, "normalize_planes: null pointer");
+  if (nchan < 1 || nchan > APX_MAX_CHAN)
+    return set_error(APX_E_INVALID, "normalize_planes: nchan out of range");
+  if (npix < 1) return set_error(APX_E_INVALID, "normalize_planes: no pixels");
+  hipLaunchKernelGGL(normalize_planes_kernel, dim3(1), dim3(1024), 0, stream, planes, nchan, npix);
+  return check_launch("normalize_planes");
+}
 
 int64_t render_workspace_bytes(int64_t npix) {
   return 256 + align256(8 * 8 * init_blocks(npix)) + align256(4 * npix) + align256(4 * npix) +

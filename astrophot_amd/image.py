@@ -530,6 +530,16 @@ class PSF_Image(Image):
     def normalize(self):
         self.data /= torch.sum(self.data)
 
+    def jacobian_image(self, parameters=None, data=None):
+        """psf_image.py:106-130."""
+        return Jacobian_Image(parameters=[] if parameters is None else parameters, data=data,
+                              window=self.window.copy())
+
+    def model_image(self, window=None):
+        """psf_image.py:132-141: a blank image on this PSF's window."""
+        w = self.window.copy() if window is None else window.copy()
+        return Model_Image(window=w)
+
     @property
     def psf_border_int(self):
         """psf_image.py:71-93: ceil((1 + shape)/2) in (x, y) order."""

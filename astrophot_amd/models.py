@@ -29,7 +29,8 @@ from .image import Jacobian_Image, Model_Image, PSF_Image, Target_Image, Window
 from .param import Param_Mask, Parameter_Node
 
 __all__ = ["AstroPhot_Model", "Component_Model", "Galaxy_Model", "Sersic_Galaxy", "Exponential_Galaxy",
-           "Gaussian_Galaxy", "Moffat_Galaxy", "Sky_Model", "Flat_Sky", "Group_Model", "differentiable_sample"]
+           "Gaussian_Galaxy", "Moffat_Galaxy", "Sky_Model", "Flat_Sky", "PSF_Model", "Moffat_PSF",
+           "Group_Model", "differentiable_sample"]
 
 
 def _all_subclasses(cls):
@@ -236,10 +237,26 @@ class Component_Model(AstroPhot_Model):
 
     @psf.setter
     def psf(self, val):
+        """model_object.py:158-174: a PSF_Image, a PSF model (re-rendered at
+        every sample, its parameters joining this model's), or pixel data."""
         if val is None or isinstance(val, PSF_Image):
             self._psf = val
+        elif isinstance(val, PSF_Model):
+            self.set_aux_psf(val)
         else:
             self._psf = PSF_Image(data=val, pixelscale=self.target.pixelscale)
+
+    def set_aux_psf(self, aux_psf, add_parameters=True):
+        """model_object.py:133-147: ``aux_psf`` is sampled at every render of
+        this model; with ``add_parameters`` its parameter node joins this
+        model's tree (ahead of the model's own parameters when set at
+        construction), so fits and Jacobians see the PSF parameters too."""
+        self._psf = aux_psf
+        if add_parameters:
+            self.parameters.link(aux_psf.parameters)
+
+    def _aux_psf(self):
+        return self._psf if isinstance(self._psf, PSF_Model) else None
 
     # -- parameter bookkeeping for the engine ------------------------------
     def _values(self):
@@ -253,14 +270,21 @@ class Component_Model(AstroPhot_Model):
         return _engine.PROFILE_SLOTS[self._profile][name]
 
     def _free_slots(self):
-        """(slot, dval/drep) for every element of the flat parameter vector."""
+        """(slot, dval/drep) for every element of the flat parameter vector.
+        Parameters of an auxiliary PSF model take slot PSF_SLOT0 + their slot
+        in the PSF model."""
+        aux = self._aux_psf()
+        aux_ids = set() if aux is None else {id(n) for n in aux.parameters._leaves()}
         out = []
         for leaf in self.parameters._leaves():
             rep = leaf._leaf_val_to_rep(leaf.value)
             d = leaf._leaf_dval_drep(rep).reshape(-1)
             for e in range(leaf.size):
                 if bool(leaf.mask.reshape(-1)[e]):
-                    out.append((self._slot_of(leaf.name, e), float(d[e])))
+                    if id(leaf) in aux_ids:
+                        out.append((_engine.PSF_SLOT0 + aux._slot_of(leaf.name, e), float(d[e])))
+                    else:
+                        out.append((self._slot_of(leaf.name, e), float(d[e])))
         return out
 
     # -- sampling (model_object.py:258-375) ---------------------------------
@@ -353,7 +377,7 @@ class Component_Model(AstroPhot_Model):
         if save_params:
             state["parameters"] = self.parameters.get_state()
         state["target_identity"] = self._target_identity
-        if isinstance(self._psf, PSF_Image):
+        if isinstance(self._psf, (PSF_Image, PSF_Model)):
             state["psf"] = self._psf.get_state()
         for key in self.track_attrs:
             if getattr(self, key) != getattr(self.__class__, key):
@@ -380,9 +404,12 @@ class Component_Model(AstroPhot_Model):
         if missing:
             raise UnrecognizedModel(f"saved state of {self.name} lacks parameters {missing}")
         if "psf" in state:
-            if state["psf"].get("type", "AstroPhot_Model") != "PSF_Image":
-                raise NotImplementedError("a saved PSF model is not supported; save the PSF as an image")
-            self.psf = PSF_Image(state=state["psf"])
+            if state["psf"].get("type", "AstroPhot_Model") == "PSF_Image":
+                self.psf = PSF_Image(state=state["psf"])
+            else:  # _model_methods.py:473-482: the PSF model reads its branch of this tree
+                sub = dict(state["psf"])
+                sub["parameters"] = self.parameters[sub["name"]]
+                self.set_aux_psf(AstroPhot_Model(name=sub["name"], filename=sub))
         return state
 
 
@@ -515,6 +542,106 @@ class Flat_Sky(Sky_Model):
     parameter_specs = {"F": {"units": "log10(flux/arcsec^2)"}}
     _parameter_order = Sky_Model._parameter_order + ("F",)
     _profile = "flat_sky"
+    usable = True
+
+
+class PSF_Model(Component_Model):
+    """psf_model_object.py:20-264: a model of the PSF itself.  Its target is a
+    PSF_Image and its centre is fixed at the plane origin (the PSF image's
+    centre pixel); it is sampled with Simpson's rule plus threshold
+    integration by default, never convolved, and normalised to unit sum over
+    the sampled window (``normalize_psf``)."""
+
+    parameter_specs = {"center": {"units": "arcsec", "value": (0.0, 0.0), "uncertainty": (0.1, 0.1),
+                                  "locked": True}}
+    _parameter_order = ("center",)
+    model_type = f"psf {AstroPhot_Model.model_type}"
+    usable = False
+    _is_psf = True
+    normalize_psf = True
+    sampling_mode = "simpsons"
+    sampling_tolerance = 1e-3
+    integrate_mode = "threshold"
+    track_attrs = ["sampling_mode", "sampling_tolerance", "integrate_mode", "integrate_max_depth",
+                   "integrate_gridding", "integrate_quad_level", "jacobian_chunksize", "softening"]
+
+    def __init__(self, *, name=None, **kwargs):
+        if "normalize_psf" in kwargs:
+            self.normalize_psf = bool(kwargs["normalize_psf"])
+        super().__init__(name=name, **kwargs)
+        if "filename" not in kwargs and self._window is None and self.target is None:
+            raise InvalidTarget("a PSF model needs a PSF_Image target or a window")
+        c = self.window.center
+        if float(c.abs().max()) > 1e-9 * float(self.window.pixel_length.max()):
+            raise SpecificationConflict("PSF models must always be centered at (0,0)")
+
+    @property
+    def target(self):
+        return self._target
+
+    @target.setter
+    def target(self, tar):
+        if not (tar is None or isinstance(tar, PSF_Image)):
+            raise InvalidTarget("PSF_Model target must be a PSF_Image instance.")
+        self._target = tar
+        if tar is not None:
+            self._target_identity = tar.identity
+
+    @property
+    def psf_mode(self):
+        return "none"
+
+    @psf_mode.setter
+    def psf_mode(self, val):
+        if val != "none":
+            raise SpecificationConflict("a PSF model is never convolved")
+
+    @property
+    def psf(self):
+        return None
+
+    @psf.setter
+    def psf(self, val):
+        if val is not None:
+            raise SpecificationConflict("a PSF model is never convolved")
+
+    def make_model_image(self, window: Optional[Window] = None):
+        """psf_model_object.py:167-178."""
+        w = self.window if window is None else (self.window & window)
+        return Model_Image(window=w.copy())
+
+    def fit_mask(self):
+        return torch.zeros((self.window.ny, self.window.nx), dtype=torch.bool, device=AP_config.ap_device)
+
+    def vjp(self, G, parameters: Optional[torch.Tensor] = None, as_representation: bool = True,
+            window: Optional[Window] = None):
+        """sum over the PSF image of G * d image / d parameter.  A PSF image is
+        small (at most 129 x 129 pixels), so this contracts its Jacobian, the
+        unit-sum normalisation included."""
+        if window is not None and not window == self.window:
+            raise SpecificationConflict("a PSF model's vjp covers its whole window")
+        jac = self.jacobian(parameters=parameters, as_representation=as_representation).data
+        G = torch.as_tensor(G, device=jac.device).to(torch.float64)
+        return torch.einsum("yxk,yx->k", jac.double(), G)
+
+    @property
+    def psf_border_int(self):
+        """The convolution border of the PSF this model renders (psf_image.py:71-93)."""
+        return (int(math.ceil((1 + self.window.nx) / 2)), int(math.ceil((1 + self.window.ny) / 2)))
+
+
+class Moffat_PSF(PSF_Model):
+    """moffat_model.py:71-119: I(R) = 10**I0 / (1 + (R/Rd)^2)^n on the PSF
+    plane, R = sqrt(x^2 + y^2 + softening^2)."""
+
+    model_type = f"moffat {PSF_Model.model_type}"
+    parameter_specs = {
+        "n": {"units": "none", "limits": (0.1, 10), "uncertainty": 0.05},
+        "Rd": {"units": "arcsec", "limits": (0, None)},
+        "I0": {"units": "log10(flux/arcsec^2)", "value": 0.0, "locked": True},
+    }
+    _parameter_order = PSF_Model._parameter_order + ("n", "Rd", "I0")
+    _profile = "moffat"
     usable = True
 
 

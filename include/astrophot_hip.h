@@ -70,7 +70,9 @@ extern "C" {
 typedef struct apx_component {
   int32_t profile;        /* APX_SERSIC .. APX_FLAT_SKY */
   int32_t sampling;       /* APX_SAMPLE_MIDPOINT | APX_SAMPLE_QUAD */
-  int32_t sample_quad;    /* N of sampling_mode "quad:N" */
+  int32_t sample_quad;    /* N of sampling_mode "quad:N"; "simpsons" (_model_methods.py:
+                             97-109) is N = 3 with sq_node -1, 0, 1 and sq_weight
+                             1/3, 4/3, 1/3: the 3 x 3 Simpson rule on the pixel */
   int32_t integrate;      /* 0 = integrate_mode "none", 1 = "threshold" */
   int32_t quad_level;     /* integrate_quad_level */
   int32_t gridding;       /* integrate_gridding */
@@ -130,6 +132,29 @@ int apx_render_selected_count(const void* workspace, void* stream, int32_t* coun
 int apx_psf_shift(const float* psf, int32_t ph, int32_t pw, double shift_x, double shift_y,
                   const double* pinv /* host, 4 doubles */, double dvdr_x, double dvdr_y, float* kernels,
                   void* stream);
+
+/* PSF_Model.sample's normalisation to unit sum (models/psf_model_object.py:
+ * 255-257) on render planes (nchan x npix float32, as apx_render_component
+ * writes them): plane 0 becomes P / S with S = sum P, and plane c >= 1
+ * (dP / d rep) becomes dP / S - P (sum dP) / S^2, the derivative of the
+ * normalised PSF.  float64 sums in a fixed order on one workgroup. */
+int apx_normalize_planes(float* planes, int32_t nchan, int64_t npix, void* stream);
+
+/* Kernels of a PSF that is rendered from parameters (an auxiliary PSF model,
+ * models/model_object.py:133-147 and 305-310, through _model_methods.py:
+ * 187-207 and 243).  T is the PSF after the sub-pixel shift: the 1-px
+ * zero-padded bilinear shift of apx_psf_shift (shifted = 1, kernel side
+ * (ph+2) x (pw+2)) or the PSF itself (shifted = 0, psf_subpixel_shift "none",
+ * side ph x pw).  K = T / sum T and, for each derivative plane dP of dpsf
+ * (nd x ph x pw), dK = dT / sum T - T (sum dT) / (sum T)^2 with dT the same
+ * shift of dP.  `out` receives nd + 2 planes [K, dK_0 .. dK_(nd-1), 0], each
+ * flipped for direct correlation like apx_psf_shift's, so that apx_convolve
+ * with kernels = out + 2k planes and dk_col = {c(2k), c(2k+1)} writes the
+ * columns of derivative planes 2k and 2k+1 (the trailing zero plane pads an
+ * odd nd).  nd <= APX_NSLOT. */
+int apx_psf_kernels_dparam(const float* psf, const float* dpsf, int32_t nd, int32_t ph, int32_t pw,
+                           int32_t shifted, double shift_x, double shift_y, float* out,
+                           void* stream);
 
 /* Direct convolution of render planes, cropped to an output box. */
 typedef struct apx_conv_job {

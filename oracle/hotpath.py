@@ -33,6 +33,12 @@ fft convolution            scipy ``fftconvolve(mode="same")`` (reference KAT
                            ``test_utils.py:15-38``) and direct conv2d
 component_sample           flux conservation, whole-pixel translation, q=1
                            PA-independence, symmetry invariants
+simpsons sampling          exact on cubics (4-point Gauss-Legendre of the
+                           same parallelogram pixel); centre sample = value
+psf_sample                 60 x 60 brute-force pixel integral of the Moffat
+                           PSF, unit sum, transpose/flip symmetry
+auxiliary PSF model        equals the galaxy convolved with the sampled PSF
+                           image; PSF-parameter columns vs central FD
 jacobian (forward AD)      central finite differences, reverse-mode autograd
 chi2                       reference KATs ``test_utils.py:55-121`` (4.5, 0.75,
                            5, 5/7, 9, 1.5, 10, 10/7)
@@ -142,6 +148,9 @@ SPECS = {
     "gaussian": _GAL + [("sigma", 1, 0.0, None, False), ("flux", 1, None, None, False)],
     "moffat": _GAL + [("n", 1, 0.1, 10.0, False), ("Rd", 1, 0.0, None, False), ("I0", 1, None, None, False)],
     "flat_sky": [("center", 2, None, None, False), ("F", 1, None, None, False)],
+    # psf_model_object.py:41-49 + moffat_model.py:94-99 (centre and I0 locked by default)
+    "moffat_psf": [("center", 2, None, None, False), ("n", 1, 0.1, 10.0, False), ("Rd", 1, 0.0, None, False),
+                   ("I0", 1, None, None, False)],
 }
 
 
@@ -199,6 +208,16 @@ class Grid:
         c = self.P @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.ref_ij.view(2, 1))
         c = c + self.ref_xy.view(2, 1)
         return c.reshape(2, self.ny, self.nx)
+
+    def simps_meshgrid(self):
+        """window_object.py:429-452: plane coords of every pixel's corners, edge
+        midpoints and centre, (2, 2 ny + 1, 2 nx + 1)."""
+        xs = 0.5 * torch.arange(2 * self.nx + 1, dtype=F64) - 0.5
+        ys = 0.5 * torch.arange(2 * self.ny + 1, dtype=F64) - 0.5
+        mi, mj = torch.meshgrid(xs, ys, indexing="xy")
+        c = self.P @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.ref_ij.view(2, 1))
+        c = c + self.ref_xy.view(2, 1)
+        return c.view(2, *mi.shape)
 
     @property
     def origin(self):
@@ -327,6 +346,8 @@ def grid_integrate(X, Y, P, evalf, quad_level=3, gridding=5, depth=1, max_depth=
 # operations.py:9-36
 # ---------------------------------------------------------------------------
 CURVATURE = torch.tensor([[0.0, 1.0, 0.0], [1.0, -4.0, 1.0], [0.0, 1.0, 0.0]], dtype=F64)
+# interpolate.py:346-355
+SIMPSONS = torch.tensor([[1.0, 4.0, 1.0], [4.0, 16.0, 4.0], [1.0, 4.0, 1.0]], dtype=F64) / 36.0
 
 
 def curvature(mid):
@@ -398,7 +419,7 @@ class Component:
     window: Grid
     values: Dict[str, torch.Tensor]
     locked: Sequence[str] = ()
-    psf: Optional[torch.Tensor] = None
+    psf: Optional[object] = None  # PSF image tensor, or a PSF-model Component
     psf_mode: str = "none"
     psf_convolve_mode: str = "fft"
     sampling_mode: str = "midpoint"
@@ -410,8 +431,15 @@ class Component:
     softening: float = 1e-3
     image_chunksize: int = 1000
 
+    normalize_psf: bool = True
+
     def __post_init__(self):
         self.values = {k: torch.as_tensor(v, dtype=F64).clone() for k, v in self.values.items()}
+        if self.kind.endswith("_psf"):
+            # psf_model_object.py:41-49: centre at the plane origin; never convolved
+            self.values.setdefault("center", torch.zeros(2, dtype=F64))
+            self.values.setdefault("I0", torch.tensor(0.0, dtype=F64))
+            self.psf_mode = "none"
         if self.kind == "flat_sky":
             # sky_model_object.py:12-35: centre locked, no psf, no integration
             self.locked = tuple(set(self.locked) | {"center"})
@@ -422,14 +450,27 @@ class Component:
                     torch.tensor([(self.window.nx - 1) / 2, (self.window.ny - 1) / 2], dtype=F64))
 
     @property
+    def aux(self):
+        """An auxiliary PSF model (model_object.py:133-147): its free
+        parameters come first in this component's vector (it is linked into
+        the parameter tree before the model's own parameters are built)."""
+        return self.psf if isinstance(self.psf, Component) else None
+
+    @property
     def free(self):
         return [s for s in SPECS[self.kind] if s[0] not in self.locked]
 
     def vector_values(self):
-        return torch.cat([self.values[s[0]].reshape(-1) for s in self.free])
+        own = [self.values[s[0]].reshape(-1) for s in self.free]
+        if self.aux is not None:
+            own = [self.aux.vector_values()] + own
+        return torch.cat(own) if own else torch.zeros(0, dtype=F64)
 
     def val_to_rep(self, vals):
         out, k = [], 0
+        if self.aux is not None:
+            k = self.aux.nparams
+            out.append(self.aux.val_to_rep(vals[:k]))
         for name, size, lo, hi, cyc in self.free:
             v = vals[k:k + size]
             k += size
@@ -442,9 +483,13 @@ class Component:
         return torch.cat(out)
 
     def rep_to_values(self, rep):
-        """parameter.py:330-374; returns a name -> value dict (locked ones fixed)."""
+        """parameter.py:330-374; returns a name -> value dict (locked ones
+        fixed); an auxiliary PSF model's values sit under "psf"."""
         vals = dict(self.values)
         k = 0
+        if self.aux is not None:
+            k = self.aux.nparams
+            vals["psf"] = self.aux.rep_to_values(rep[:k])
         for name, size, lo, hi, cyc in self.free:
             r = rep[k:k + size]
             k += size
@@ -459,7 +504,7 @@ class Component:
 
     @property
     def nparams(self):
-        return sum(s[1] for s in self.free)
+        return sum(s[1] for s in self.free) + (self.aux.nparams if self.aux is not None else 0)
 
     # -- evaluate_model ---------------------------------------------------
     def evaluate(self, X, Y, P, v):
@@ -467,6 +512,10 @@ class Component:
         area = torch.linalg.det(P).abs()
         if self.kind == "flat_sky":
             return torch.ones_like(X) * (area * 10 ** v["F"])
+        if self.kind.endswith("_psf"):
+            # psf_model_object.py:136-137 (no transform) + _shared_methods.py:275-283
+            R = torch.sqrt(X**2 + Y**2 + self.softening**2)
+            return radial(self.kind[:-4], R, v, area)
         theta = -(v["PA"] - NORTH)
         s, c = torch.sin(theta), torch.cos(theta)
         Xr, Yr = c * X - s * Y, s * X + c * Y
@@ -493,6 +542,12 @@ def sample_init(comp, grid, v, center):
         n = int(comp.sampling_mode.split(":")[1])
         res, ref = single_quad_integrate(X, Y, grid.P, ev, n)
         return ref, res
+    if comp.sampling_mode == "simpsons":  # _model_methods.py:97-109
+        S = grid.simps_meshgrid()
+        dens = ev(S[0] - center[0], S[1] - center[1], grid.P)
+        mid = dens[1::2, 1::2]
+        simps = F.conv2d(dens.view(1, 1, *dens.shape), SIMPSONS.view(1, 1, 3, 3), stride=2, padding="valid")
+        return mid, simps.squeeze(0).squeeze(0)
     raise ValueError(f"unknown sampling mode {comp.sampling_mode}")
 
 
@@ -513,11 +568,26 @@ def sample_integrate(comp, deep, reference, grid, v, center):
     return deep
 
 
+def psf_sample(comp, v, sample_window=None):
+    """PSF_Model.sample with image=None (psf_model_object.py:180-264): sampled
+    on the window, integrated around the origin, normalised to unit sum.
+    Returns (data, grid)."""
+    working = comp.window.copy() if sample_window is None else (comp.window & sample_window)
+    reference, deep = sample_init(comp, working, v, v["center"])
+    deep = sample_integrate(comp, deep, reference, working, v, torch.zeros(2, dtype=F64))
+    if comp.normalize_psf:
+        deep = deep / torch.sum(deep)
+    return deep, working
+
+
 def component_sample(comp, v, sample_window=None):
     """Component_Model.sample with image=None (model_object.py:258-375).
 
     Returns (data, grid) over ``comp.window & sample_window`` (or comp.window).
+    An auxiliary PSF model is sampled first (model_object.py:305-310).
     """
+    if comp.kind.endswith("_psf"):
+        return psf_sample(comp, v, sample_window)
     if sample_window is None:
         img_grid = comp.window.copy()
         working = comp.window.copy()
@@ -527,6 +597,8 @@ def component_sample(comp, v, sample_window=None):
     center = v["center"]
     if comp.psf_mode == "full":
         psf = comp.psf
+        if comp.aux is not None:
+            psf, _ = psf_sample(comp.aux, v["psf"])
         bx, by = psf_border(psf)
         working.pad(bx, by)
         pc = working.plane_to_pixel(center)

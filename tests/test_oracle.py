@@ -259,6 +259,96 @@ def test_jacobian_vs_central_fd_and_reverse_mode():
     assert torch.allclose(g, (J * wgt[..., None]).sum(dim=(0, 1)), rtol=1e-10)
 
 
+# -- PSF models: psf_model_object.py:180-264, _model_methods.py:97-109 ---------
+class _PolyComp:
+    """Stand-in component whose brightness is a polynomial in (x, y)."""
+
+    sampling_mode = "simpsons"
+
+    def __init__(self, a, b):
+        self.a, self.b = a, b
+
+    def evaluate(self, X, Y, P, v):
+        return (X + 0.3) ** self.a * (Y - 0.2) ** self.b + 0.5 * X * Y
+
+
+@pytest.mark.parametrize("P", [0.8, [[0.7, 0.2], [-0.1, 0.9]]])
+def test_simpsons_sampling_exact_on_cubics(P):
+    """Simpson's 3 x 3 rule on each (parallelogram) pixel integrates a cubic
+    exactly: checked against 4-point Gauss-Legendre quadrature of the same
+    pixel; the reference sample is the pixel-centre value."""
+    grid = H.Grid(5, 4, P, ref_ij=(1.3, 0.4), ref_xy=(0.2, -0.7))
+    xg, wg = roots_legendre(4)
+    # degree <= 3 in each pixel coordinate: any total degree <= 3, and x^a y^b
+    # with a, b <= 3 when the pixel axes are the plane axes
+    cases = [(0, 0), (1, 2), (3, 0), (2, 1), (0, 3)] + ([(2, 3), (3, 3)] if np.ndim(P) == 0 else [])
+    for a, b in cases:
+        comp = _PolyComp(a, b)
+        mid, simps = H.sample_init(comp, grid, {}, torch.zeros(2, dtype=F64))
+        C = grid.meshgrid()
+        Pm = grid.P
+        exact = torch.zeros_like(mid)
+        for ux, wx in zip(xg / 2, wg / 2):
+            for uy, wy in zip(xg / 2, wg / 2):
+                dx = Pm[0, 0] * ux + Pm[0, 1] * uy
+                dy = Pm[1, 0] * ux + Pm[1, 1] * uy
+                exact += wx * wy * comp.evaluate(C[0] + dx, C[1] + dy, Pm, None)
+        assert torch.allclose(simps, exact, rtol=1e-12, atol=1e-12), (a, b)
+        assert torch.allclose(mid, comp.evaluate(C[0], C[1], Pm, None), rtol=1e-14, atol=1e-14)
+
+
+def _psf_comp(n=2.5, Rd=2.2, side=15, P=1.0, **kw):
+    g = H.Grid(side, side, P, ref_ij=((side - 1) / 2, (side - 1) / 2), ref_xy=(0.0, 0.0))
+    kw.setdefault("sampling_mode", "simpsons")
+    kw.setdefault("sampling_tolerance", 1e-3)
+    return H.Component("moffat_psf", g, {"n": n, "Rd": Rd}, locked=("center", "I0"), **kw)
+
+
+@pytest.mark.parametrize("n,Rd", [(2.5, 2.2), (1.2, 0.9), (4.0, 5.0)])
+def test_psf_sample_brute_force(n, Rd):
+    """A sampled Moffat PSF model is the normalised pixel integral of
+    (1 + (r/Rd)^2)^-n: against 60 x 60 midpoint sub-sampling of every pixel,
+    unit sum, and the symmetries of a centred round profile."""
+    c = _psf_comp(n, Rd)
+    m = H.Model([c])
+    img = m.forward(m.vector_representation())
+    assert img.sum().item() == pytest.approx(1.0, abs=1e-13)
+    assert torch.allclose(img, img.T, rtol=1e-12) and torch.allclose(img, img.flip(0), rtol=1e-12)
+    k = 60
+    u = (torch.arange(k, dtype=F64) + 0.5) / k - 0.5
+    xs = (torch.arange(15, dtype=F64) - 7)[:, None] + u[None, :]
+    X, Y = xs.reshape(-1)[None, :], xs.reshape(-1)[:, None]
+    f = (1 + (X**2 + Y**2 + 1e-6) / Rd**2) ** -n
+    brute = f.reshape(15, k, 15, k).mean(dim=(1, 3))
+    brute = brute / brute.sum()
+    assert (img - brute).abs().max().item() < 2e-4 * brute.max().item()
+
+
+def test_aux_psf_model_matches_image_psf_and_fd():
+    """A galaxy convolved with an auxiliary PSF model renders exactly as with
+    that model's sampled image as a fixed PSF, and its Jacobian, the PSF
+    parameters' columns included (first in the vector), matches central
+    finite differences."""
+    aux = _psf_comp(2.2, 1.8, side=9)
+    psf_img, _ = H.psf_sample(aux, aux.rep_to_values(aux.val_to_rep(aux.vector_values())))
+    a = _comp("sersic", (20.3, 17.6), psf=aux, psf_mode="full")
+    b = _comp("sersic", (20.3, 17.6), psf=psf_img, psf_mode="full")
+    ma, mb = H.Model([a]), H.Model([b])
+    ra = ma.vector_representation()
+    assert len(ra) == len(mb.vector_representation()) + 2
+    assert torch.equal(ra[2:], mb.vector_representation())
+    assert torch.equal(ma.forward(ra), mb.forward(mb.vector_representation()))
+    J = ma.jacobian(ra)
+    h = 1e-5
+    for k in range(len(ra)):
+        r1, r2 = ra.clone(), ra.clone()
+        r1[k] += h
+        r2[k] -= h
+        fd = (ma.forward(r1) - ma.forward(r2)) / (2 * h)
+        assert (fd - J[..., k]).abs().max() <= 1e-6 * J[..., k].abs().max() + 1e-12, k
+    assert J[..., 0].abs().max() > 0 and J[..., 1].abs().max() > 0
+
+
 # -- chi^2 KATs: test_utils.py:55-121 ------------------------------------------
 def test_chi2_kats():
     ones, zeros = torch.ones(10, dtype=F64), torch.zeros(10, dtype=F64)

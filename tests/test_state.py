@@ -153,3 +153,45 @@ def test_unknown_format_and_model_type(tmp_path):
     with pytest.raises(ap.errors.UnrecognizedModel):
         AstroPhot_Model(name="x", filename=st)
     assert json.loads(json.dumps(m.get_state()))["model_type"] == "sersic galaxy model"
+
+
+def test_group_with_psf_model_state(tmp_path):
+    """The reference's group save/load test (tests/test_group_models.py:112-165):
+    a Moffat PSF model (n, Rd free) as the first model's PSF gives 16
+    parameters, its node leading that model's branch; all of it survives
+    save and load."""
+    tar = target(51, 51, 1.0)
+    psf = AstroPhot_Model(name="psf model 1", model_type="moffat psf model",
+                          target=ap.image.PSF_Image(data=np.zeros((11, 11)), pixelscale=1.0),
+                          parameters={"n": 2.0, "Rd": 3.0})
+    mod1 = sersic("base model 1", tar, psf=psf, psf_mode="full")
+    mod2 = sersic("base model 2", tar)
+    smod = AstroPhot_Model(name="group model", model_type="group model", models=[mod1, mod2], target=tar)
+    assert smod.parameters.size == 16
+    assert list(mod1.parameters.vector_names())[:2] == ["n", "Rd"]
+    fn = str(tmp_path / "g.yaml")
+    smod.save(fn)
+    new = AstroPhot_Model(name="group model", filename=fn)
+    assert len(new.models) == 2 and new.parameters.size == 16
+    assert torch.equal(new.parameters.vector_values(), smod.parameters.vector_values())
+    p1 = new.models["base model 1"].psf
+    assert type(p1).__name__ == "Moffat_PSF" and p1.window.get_state() == psf.window.get_state()
+    assert p1.parameters is new.models["base model 1"].parameters["psf model 1"]
+    assert p1["I0"].locked and p1["center"].locked
+    assert [s for s, _ in new.models["base model 1"]._free_slots()] == \
+        [s for s, _ in mod1._free_slots()]
+
+
+def test_psf_model_must_be_centred_and_unconvolved():
+    img = ap.image.PSF_Image(data=np.zeros((9, 9)), pixelscale=0.5)
+    m = AstroPhot_Model(name="p", model_type="moffat psf model", target=img, parameters={"n": 2.0, "Rd": 1.0})
+    assert m.psf_mode == "none" and m.sampling_mode == "simpsons" and m.sampling_tolerance == 1e-3
+    assert m.parameters.size == 2 and m.window.nx == 9
+    with pytest.raises(ap.errors.SpecificationConflict):
+        m.psf_mode = "full"
+    with pytest.raises(ap.errors.InvalidTarget):
+        AstroPhot_Model(name="q", model_type="moffat psf model", target=target(), parameters={"n": 2.0, "Rd": 1.0})
+    with pytest.raises(ap.errors.SpecificationConflict):
+        AstroPhot_Model(name="r", model_type="moffat psf model", target=img,
+                        window=ap.image.Window(pixel_shape=(9, 9), pixelscale=0.5, origin=(0.0, 0.0)),
+                        parameters={"n": 2.0, "Rd": 1.0})

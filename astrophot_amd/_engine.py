@@ -149,8 +149,10 @@ class Plan:
     chan_slots: List[int]
 
 
-def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Plan:
-    """Fill the apx_component of ``model`` for sampling ``window``."""
+def plan(model, values, window, slot_cols=None, dvdr=None, channels=False, ref_npix=None) -> Plan:
+    """Fill the apx_component of ``model`` for sampling ``window``.
+    ``ref_npix`` (Sersic): the working-grid pixel count of the window the
+    model is nominally sampled on, when only a part of it is rendered."""
     kind = model._profile
     psf_full = "full" in model.psf_mode and kind != "flat_sky"
     window = window.copy()
@@ -238,7 +240,7 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False) -> Pl
     if kind == "sersic":
         cp.mean_reference = 0
         cp.reference = sersic_total_flux(10 ** float(values["Ie"]), float(values["n"]),
-                                         float(values["Re"]), float(values["q"])) / (nx * ny)
+                                         float(values["Re"]), float(values["q"])) / (ref_npix or nx * ny)
     else:
         cp.mean_reference = 1
     # Jacobian / adjoint channels: the centre acts through the shifted PSF when
@@ -349,6 +351,36 @@ def _new_job(pl, kh, kw):
     return job
 
 
+def _sampled_part(model, req: RenderRequest):
+    """The window actually rendered, and the Sersic threshold's pixel count.
+
+    A component of a group is sampled on the whole group window and kept on
+    its own (group_model_object.py:183-231, model_object.py:258-375).  On the
+    kept pixels that equals sampling just the kept region plus what reaches
+    it: the PSF border (psf_mode "full") or one pixel for the curvature
+    stencil, clipped to the window, as long as the integration threshold
+    still divides the Sersic flux by the whole window's working-grid pixel
+    count (sersic_model.py:87-89).  Other profiles take the mean of the whole
+    grid as the threshold reference and are rendered on the whole window."""
+    region = req.region
+    w = req.window
+    if region is None or model._profile != "sersic" or getattr(model, "_is_psf", False) or req.jac is not None:
+        return w, None
+    part = region & w
+    if part.nx * part.ny >= w.nx * w.ny:
+        return w, None
+    if "full" in model.psf_mode:
+        bx, by = model.psf.psf_border_int
+    else:
+        part = part.copy()
+        part.pad_pixel((1, 1))
+        part = part & w
+        bx = by = 0
+    if part.nx < 3 or part.ny < 3:
+        return w, None
+    return part, (w.nx + 2 * bx) * (w.ny + 2 * by)
+
+
 def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     """Render ``model`` with parameter ``values`` (host float64) on
     ``req.window``; accumulate into ``req.image`` / ``req.jac``."""
@@ -357,7 +389,8 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     all_cols = req.slot_cols or {}
     slot_cols = {s: c for s, c in all_cols.items() if s < PSF_SLOT0}
     psf_cols = {s - PSF_SLOT0: c for s, c in all_cols.items() if s >= PSF_SLOT0}
-    pl = plan(model, values, req.window, slot_cols, dvdr, channels=req.jac is not None)
+    sample_window, ref_npix = _sampled_part(model, req)
+    pl = plan(model, values, sample_window, slot_cols, dvdr, channels=req.jac is not None, ref_npix=ref_npix)
     cp, nx, ny = pl.cp, pl.cp.nx, pl.cp.ny
     dev = AP_config.ap_device
     planes = _buffer(model, "planes", 4 * cp.nchan * ny * nx, dev)
@@ -371,7 +404,6 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     kern = _kernels(model, pl, dvdr, dev, st, sorted(psf_cols) if req.jac is not None else ())
     kptr, kh, kw = kern.ptr, kern.kh, kern.kw
     # crop back to the sampling window, then window-aware accumulation
-    sample_window = req.window
     cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
     cws = _buffer(model, "conv_ws", cwsb, dev)
     job = _new_job(pl, kh, kw)

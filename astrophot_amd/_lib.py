@@ -59,6 +59,17 @@ class ConvJob(ctypes.Structure):
     ]
 
 
+class Group(ctypes.Structure):
+    """apx_group."""
+
+    _fields_ = [
+        ("nx", _i32), ("ny", _i32), ("nsrc", _i32), ("np", _i32), ("sky_col", _i32), ("sky_after", _i32),
+        ("ntx", _i32), ("nty", _i32), ("npair", _i32), ("reserved", _i32),
+        ("win", _p), ("pix_off", _p), ("col0", _p), ("tile_ptr", _p), ("tile_src", _p), ("pair", _p),
+        ("nsrcpix", _i64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/astrophot_hip.h
 SIGNATURES = {
     "apx_last_error": (ctypes.c_char_p, []),
@@ -68,6 +79,13 @@ SIGNATURES = {
     "apx_render_selected_count": (ctypes.c_int, [_p, _p, ctypes.POINTER(_i32)]),
     "apx_psf_shift": (ctypes.c_int, [_p, _i32, _i32, _f64, _f64, _p, _f64, _f64, _p, _p]),
     "apx_normalize_planes": (ctypes.c_int, [_p, _i32, _i64, _p]),
+    "apx_group_workspace_bytes": (_i64, [ctypes.POINTER(Group)]),
+    "apx_group_image": (ctypes.c_int, [ctypes.POINTER(Group), _p, _f64, _p, _p]),
+    "apx_group_normal": (ctypes.c_int, [ctypes.POINTER(Group), _p, _p, _p, _p, _p, _f64, _p, _p, _p, _i64, _p]),
+    "apx_group_rpp": (ctypes.c_int, [ctypes.POINTER(Group), _p, _p, _p, _p, _p, _p, _f64, _f64, _p, _p, _i64,
+                                     _p]),
+    "apx_dense_damped_factor": (ctypes.c_int, [_i32, _p, _f64, _p, _p, _p]),
+    "apx_dense_cholesky_solve": (ctypes.c_int, [_i32, _p, _p, _p, _p]),
     "apx_psf_kernels_dparam": (ctypes.c_int, [_p, _p, _i32, _i32, _i32, _i32, _f64, _f64, _p, _p]),
     "apx_convolve_workspace_bytes": (_i64, [_i32, _i32]),
     "apx_convolve": (ctypes.c_int, [ctypes.POINTER(ConvJob), _p, _p, _p, _i64, _p]),

@@ -39,7 +39,7 @@ class Galaxy(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ph", ctypes.c_int32), ("pw", ctypes.c_int32),
                 ("pix_off", ctypes.c_int64), ("psf_off", ctypes.c_int64), ("work_off", ctypes.c_int64),
                 ("tap_off", ctypes.c_int64), ("chunk_off", ctypes.c_int64), ("tile_off", ctypes.c_int64),
-                ("origin", ctypes.c_double * 2)]
+                ("origin", ctypes.c_double * 2), ("ref_npix", ctypes.c_int64)]
 
 
 class Batch(ctypes.Structure):

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, in
       const double bn = sersic_b(v[4]);
       const double flux = 2.0 * kPi * pow(10.0, v[6]) * v[5] * v[5] * v[2] * v[4] * exp(bn) *
                           pow(bn, -2.0 * v[4]) * exp(lgamma(2.0 * v[4]));  // functions.py:168-190
-      e.thr = b.tolerance * flux / ((double)nxw * nyw);
+      e.thr = b.tolerance * flux / (G.ref_npix > 0 ? (double)G.ref_npix : (double)nxw * nyw);
       for (int q = 0; q < NP; ++q) e.dvdr[q] = dvdr[q];
       e.nxw = nxw;
       e.nyw = nyw;

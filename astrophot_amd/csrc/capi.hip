@@ -79,6 +79,35 @@ int apx_convolve(const apx_conv_job* job, const float* planes, const float* kern
                        static_cast<hipStream_t>(stream));
 }
 
+int64_t apx_group_workspace_bytes(const apx_group* group) { return apx::group_workspace_bytes(group); }
+
+int apx_group_image(const apx_group* group, const float* src_model, double sky, float* out, void* stream) {
+  return apx::group_image(group, src_model, sky, out, static_cast<hipStream_t>(stream));
+}
+
+int apx_group_normal(const apx_group* group, const float* J, const float* model, const float* data,
+                     const float* weight, const uint8_t* mask, double sky_dcol, double* hess, double* grad,
+                     void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::group_normal(group, J, model, data, weight, mask, sky_dcol, hess, grad, workspace,
+                           workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int apx_group_rpp(const apx_group* group, const float* J, const float* m0, const float* m1,
+                  const float* weight, const uint8_t* mask, const double* h, double d, double sky_dcol,
+                  double* rpp, void* workspace, int64_t workspace_bytes, void* stream) {
+  return apx::group_rpp(group, J, m0, m1, weight, mask, h, d, sky_dcol, rpp, workspace, workspace_bytes,
+                        static_cast<hipStream_t>(stream));
+}
+
+int apx_dense_damped_factor(int32_t n, const double* hess, double L, double* factor, int32_t* status,
+                            void* stream) {
+  return apx::dense_damped_factor(n, hess, L, factor, status, static_cast<hipStream_t>(stream));
+}
+
+int apx_dense_cholesky_solve(int32_t n, const double* factor, const double* b, double* x, void* stream) {
+  return apx::dense_cholesky_solve(n, factor, b, x, static_cast<hipStream_t>(stream));
+}
+
 int64_t apx_reduce_workspace_bytes(int64_t npix, int32_t np) {
   return apx::reduce_workspace_bytes(npix, np);
 }

@@ -31,6 +31,17 @@ int64_t convolve_workspace_bytes(int32_t kh, int32_t kw);
 int convolve(const apx_conv_job* job, const float* planes, const float* kernels, void* workspace,
              int64_t ws_bytes, hipStream_t stream);
 
+int group_image(const apx_group* g, const float* src, double sky, float* out, hipStream_t stream);
+int64_t group_workspace_bytes(const apx_group* g);
+int group_normal(const apx_group* g, const float* J, const float* model, const float* data,
+                 const float* weight, const uint8_t* mask, double sky_dcol, double* H, double* grad,
+                 void* ws, int64_t ws_bytes, hipStream_t stream);
+int group_rpp(const apx_group* g, const float* J, const float* m0, const float* m1, const float* weight,
+              const uint8_t* mask, const double* h, double d, double sky_dcol, double* out, void* ws,
+              int64_t ws_bytes, hipStream_t stream);
+int dense_damped_factor(int32_t n, const double* H, double L, double* A, int32_t* status, hipStream_t stream);
+int dense_cholesky_solve(int32_t n, const double* A, const double* b, double* x, hipStream_t stream);
+
 int64_t reduce_workspace_bytes(int64_t npix, int32_t np);
 int chi2(const float* model, const float* data, const float* weight, const uint8_t* mask,
          int64_t npix, double* out, void* ws, int64_t ws_bytes, hipStream_t stream);

@@ -1,0 +1,613 @@
+// A group model whose sources sit on windows of one image, with block-sparse
+// Jacobian products for Levenberg-Marquardt (group_model_object.py:183-283,
+// fit/lm.py:359-406).
+//
+// Each source is one cutout of an apx_batch (its window on the image), so
+// its model and its 7 Jacobian columns live only on that window, in the
+// batch's per-source buffers.  What LM needs from the whole group is built
+// from those blocks without ever forming the dense (pixels x parameters) J:
+//   image    sky + sum of the sources covering each pixel, in group order;
+//   J^T W J  one 7 x 7 block per pair of overlapping windows, plus the sky
+//            row (the sky's column is a constant on the image);
+//   J^T v    per source over its window;  J h  per pixel over its sources.
+// Pixels find their sources through a per-tile list (64 x 16 tiles) built on
+// the host; every sum runs in a fixed order, so repeated runs give identical
+// bits.  The damped normal equations are solved by a blocked float64
+// Cholesky factorisation (right-looking, 32-column panels) and two blocked
+// triangular solves.
+#include <hip/hip_runtime.h>
+
+#include "capi_internal.h"
+
+namespace apx {
+
+namespace {
+
+constexpr int kTW = 64, kTH = 16;  // image tile
+constexpr int kNB = 32;            // Cholesky panel
+constexpr int kUT = 64;            // trailing-update tile
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of the source values covering pixel (x, y), with the sky added where
+// it stands in the group order.
+template <typename F>
+__device__ __forceinline__ double cover_sum(const apx_group& g, int x, int y, int tile, F&& value) {
+  double v = 0.0;
+  const int s0 = g.tile_ptr[tile], s1 = g.tile_ptr[tile + 1];
+  bool sky_done = g.sky_after < 0;
+  for (int e = s0; e < s1; ++e) {
+    const int s = g.tile_src[e];
+    if (!sky_done && s >= g.sky_after) {
+      v += value(-1, 0);
+      sky_done = true;
+    }
+    const int* w = g.win + 4 * s;
+    const int dx = x - w[0], dy = y - w[1];
+    if (dx >= 0 && dx < w[2] && dy >= 0 && dy < w[3])
+      v += value(s, g.pix_off[s] + (int64_t)dy * w[2] + dx);
+  }
+  if (!sky_done) v += value(-1, 0);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void group_image_kernel(apx_group g, const float* __restrict__ src,
+                                                          double sky, float* __restrict__ out) {
+  const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx;
+  const int x = tx * kTW + (threadIdx.x & 63);
+  for (int r = threadIdx.x >> 6; r < kTH; r += 4) {
+    const int y = ty * kTH + r;
+    if (x >= g.nx || y >= g.ny) continue;
+    const double v = cover_sum(g, x, y, blockIdx.x, [&](int s, int64_t p) -> double {
+      return s < 0 ? sky : (double)src[p];
+    });
+    out[(int64_t)y * g.nx + x] = (float)v;
+  }
+}
+
+// u = (2/d) (w (m1 - m0) / d - w (J h)) per pixel (zero where masked)
+__global__ __launch_bounds__(256) void group_rpp_weights_kernel(apx_group g, const float* __restrict__ J,
+                                                                const float* __restrict__ m0,
+                                                                const float* __restrict__ m1,
+                                                                const float* __restrict__ weight,
+                                                                const uint8_t* __restrict__ mask,
+                                                                const double* __restrict__ h, double d,
+                                                                double sky_dcol, double* __restrict__ u) {
+  const int tx = blockIdx.x % g.ntx, ty = blockIdx.x / g.ntx;
+  const int x = tx * kTW + (threadIdx.x & 63);
+  for (int r = threadIdx.x >> 6; r < kTH; r += 4) {
+    const int y = ty * kTH + r;
+    if (x >= g.nx || y >= g.ny) continue;
+    const int64_t p = (int64_t)y * g.nx + x;
+    const double jh = cover_sum(g, x, y, blockIdx.x, [&](int s, int64_t q) -> double {
+      if (s < 0) return g.sky_col >= 0 ? sky_dcol * h[g.sky_col] : 0.0;
+      const int c = g.col0[s];
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < APX_BATCH_NP; ++k) t += (double)J[(int64_t)k * g.nsrcpix + q] * h[c + k];
+      return t;
+    });
+    const double w = weight ? (double)weight[p] : 1.0;
+    const bool keep = !(mask && mask[p]);
+    u[p] = keep ? (2.0 / d) * (w * (double)(m1[p] - m0[p]) / d - w * jh) : 0.0;
+  }
+}
+
+// out[col0 + k] = sum over the window of J_k * v (v a double image)
+__global__ __launch_bounds__(256) void group_jtv_kernel(apx_group g, const float* __restrict__ J,
+                                                        const double* __restrict__ v,
+                                                        double* __restrict__ out) {
+  __shared__ double red[4][APX_BATCH_NP];
+  const int s = blockIdx.x;
+  const int* w = g.win + 4 * s;
+  const int64_t n = (int64_t)w[2] * w[3];
+  double acc[APX_BATCH_NP];
+#pragma unroll
+  for (int k = 0; k < APX_BATCH_NP; ++k) acc[k] = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += 256) {
+    const int dy = (int)(e / w[2]), dx = (int)(e % w[2]);
+    const double vv = v[(int64_t)(w[1] + dy) * g.nx + (w[0] + dx)];
+    const int64_t q = g.pix_off[s] + e;
+#pragma unroll
+    for (int k = 0; k < APX_BATCH_NP; ++k) acc[k] += (double)J[(int64_t)k * g.nsrcpix + q] * vv;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < APX_BATCH_NP; ++k) {
+    const double t = wsum(acc[k]);
+    if (lane == 0) red[wv][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < APX_BATCH_NP) {
+    const int k = threadIdx.x;
+    out[g.col0[s] + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+  }
+}
+
+// Image-wide float64 sums, one partial per workgroup (fixed order):
+//   mode 0: part[b] = {sum w, sum w (data - model)}   (sky row of J^T W J, J^T W r)
+//   mode 1: part[b] = {sum v, 0}                       (sky entry of J^T v)
+__global__ __launch_bounds__(256) void group_image_sums_kernel(int64_t npix, int mode,
+                                                               const float* __restrict__ model,
+                                                               const float* __restrict__ data,
+                                                               const float* __restrict__ weight,
+                                                               const uint8_t* __restrict__ mask,
+                                                               const double* __restrict__ v,
+                                                               double* __restrict__ part) {
+  __shared__ double red[2][4];
+  double a = 0.0, b = 0.0;
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < npix; p += (int64_t)gridDim.x * 256) {
+    if (mode == 1) {
+      a += v[p];
+      continue;
+    }
+    if (mask && mask[p]) continue;
+    const double w = weight ? (double)weight[p] : 1.0;
+    a += w;
+    b += w * ((double)data[p] - (double)model[p]);
+  }
+  a = wsum(a);
+  b = wsum(b);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = a;
+    red[1][wv] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2)
+    part[2 * blockIdx.x + threadIdx.x] =
+        red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+}
+
+// the sky entries from the image partials:
+//   mode 0: H[sky][sky] = c^2 sum w, grad[sky] = c sum w r;  mode 1: out[sky] = c sum v
+__global__ void group_sky_final_kernel(int nblocks, int mode, int np, int sky, double c,
+                                       const double* __restrict__ part, double* __restrict__ H,
+                                       double* __restrict__ vec) {
+  if (threadIdx.x != 0) return;
+  double a = 0.0, b = 0.0;
+  for (int k = 0; k < nblocks; ++k) {
+    a += part[2 * k];
+    b += part[2 * k + 1];
+  }
+  if (mode == 0) {
+    H[(int64_t)sky * np + sky] = c * c * a;
+    vec[sky] = c * b;
+  } else {
+    vec[sky] = c * a;
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(double* __restrict__ a, int64_t n, double v) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) a[e] = v;
+}
+
+// Per source: grad[col0 + k] = sum J_k w (data - model) over the window, and
+// the sky column H[col0 + k][sky] = H[sky][col0 + k] = c sum J_k w.
+__global__ __launch_bounds__(256) void group_source_kernel(apx_group g, const float* __restrict__ J,
+                                                           const float* __restrict__ model,
+                                                           const float* __restrict__ data,
+                                                           const float* __restrict__ weight,
+                                                           const uint8_t* __restrict__ mask, double c,
+                                                           double* __restrict__ H, double* __restrict__ grad) {
+  __shared__ double red[4][2 * APX_BATCH_NP];
+  const int s = blockIdx.x;
+  const int* w = g.win + 4 * s;
+  const int64_t n = (int64_t)w[2] * w[3];
+  double ag[APX_BATCH_NP], ah[APX_BATCH_NP];
+#pragma unroll
+  for (int k = 0; k < APX_BATCH_NP; ++k) ag[k] = ah[k] = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += 256) {
+    const int dy = (int)(e / w[2]), dx = (int)(e % w[2]);
+    const int64_t p = (int64_t)(w[1] + dy) * g.nx + (w[0] + dx);
+    if (mask && mask[p]) continue;
+    const double ww = weight ? (double)weight[p] : 1.0;
+    const double wr = ww * ((double)data[p] - (double)model[p]);
+    const int64_t q = g.pix_off[s] + e;
+#pragma unroll
+    for (int k = 0; k < APX_BATCH_NP; ++k) {
+      const double j = (double)J[(int64_t)k * g.nsrcpix + q];
+      ag[k] += j * wr;
+      ah[k] += j * ww;
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < APX_BATCH_NP; ++k) {
+    const double t0 = wsum(ag[k]), t1 = wsum(ah[k]);
+    if (lane == 0) {
+      red[wv][k] = t0;
+      red[wv][APX_BATCH_NP + k] = t1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < APX_BATCH_NP) {
+    const int k = threadIdx.x, col = g.col0[s] + k;
+    grad[col] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    if (g.sky_col >= 0) {
+      const int m = APX_BATCH_NP + k;
+      const double hs = c * (red[0][m] + red[1][m] + red[2][m] + red[3][m]);
+      H[(int64_t)col * g.np + g.sky_col] = hs;
+      H[(int64_t)g.sky_col * g.np + col] = hs;
+    }
+  }
+}
+
+// One 7 x 7 block of J^T W J per pair of overlapping windows (a <= b): sum
+// over the overlap of w J_a J_b^T; written at (a, b) and (b, a).
+__global__ __launch_bounds__(256) void group_pair_kernel(apx_group g, const float* __restrict__ J,
+                                                         const float* __restrict__ weight,
+                                                         const uint8_t* __restrict__ mask,
+                                                         double* __restrict__ H) {
+  constexpr int NP = APX_BATCH_NP, NN = NP * NP;
+  __shared__ double red[4][NN];
+  const int a = g.pair[2 * blockIdx.x], b = g.pair[2 * blockIdx.x + 1];
+  const int* wa = g.win + 4 * a;
+  const int* wb = g.win + 4 * b;
+  const int x0 = max(wa[0], wb[0]), x1 = min(wa[0] + wa[2], wb[0] + wb[2]);
+  const int y0 = max(wa[1], wb[1]), y1 = min(wa[1] + wa[3], wb[1] + wb[3]);
+  const int ow = x1 - x0, oh = y1 - y0;
+  double acc[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) acc[k] = 0.0;
+  if (ow > 0 && oh > 0) {
+    const int64_t n = (int64_t)ow * oh;
+    for (int64_t e = threadIdx.x; e < n; e += 256) {
+      const int y = y0 + (int)(e / ow), x = x0 + (int)(e % ow);
+      const int64_t p = (int64_t)y * g.nx + x;
+      if (mask && mask[p]) continue;
+      const double ww = weight ? (double)weight[p] : 1.0;
+      const int64_t qa = g.pix_off[a] + (int64_t)(y - wa[1]) * wa[2] + (x - wa[0]);
+      const int64_t qb = g.pix_off[b] + (int64_t)(y - wb[1]) * wb[2] + (x - wb[0]);
+      double ja[NP], jb[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        ja[k] = ww * (double)J[(int64_t)k * g.nsrcpix + qa];
+        jb[k] = (double)J[(int64_t)k * g.nsrcpix + qb];
+      }
+#pragma unroll
+      for (int k = 0; k < NP; ++k)
+#pragma unroll
+        for (int l = 0; l < NP; ++l) acc[k * NP + l] += ja[k] * jb[l];
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    const double t = wsum(acc[k]);
+    if (lane == 0) red[wv][k] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < NN) {
+    const int k = threadIdx.x / NP, l = threadIdx.x % NP;
+    const double v = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    const int ca = g.col0[a], cb = g.col0[b];
+    H[(int64_t)(ca + k) * g.np + (cb + l)] = v;
+    H[(int64_t)(cb + l) * g.np + (ca + k)] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dense float64 damped Cholesky: A(L)_ii = H_ii + L (1 + H_ii), A(L)_ij =
+// H_ij / (1 + L) (fit/lm.py:359-371), factor A = L L^T in place (lower
+// triangle), then solve L L^T x = b.
+
+__global__ __launch_bounds__(256) void damp_kernel(const double* __restrict__ H, int n, double L,
+                                                   double* __restrict__ A) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nn; e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double v = H[e];
+    A[e] = (i == j) ? v + L * (1.0 + v) : v / (1.0 + L);
+  }
+}
+
+// factor the diagonal block [k0, k0 + nb): one workgroup
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A, int n, int k0,
+                                                         int32_t* __restrict__ status) {
+  __shared__ double T[kNB][kNB + 1];
+  const int nb = min(kNB, n - k0);
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    T[i][j] = A[(int64_t)(k0 + i) * n + (k0 + j)];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    if (threadIdx.x == 0) {
+      const double piv = T[j][j];
+      if (!(piv > 0.0) || !isfinite(piv)) {
+        if (status[0] == 0) status[0] = k0 + j + 1;
+        T[j][j] = __builtin_nan("");
+      } else {
+        T[j][j] = sqrt(piv);
+      }
+    }
+    __syncthreads();
+    for (int i = j + 1 + (int)threadIdx.x; i < nb; i += 256) T[i][j] /= T[j][j];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nb * nb; e += 256) {
+      const int i = e / nb, k = e % nb;
+      if (k > j && i >= k) T[i][k] -= T[i][j] * T[k][j];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    if (j <= i) A[(int64_t)(k0 + i) * n + (k0 + j)] = T[i][j];
+  }
+}
+
+// panel below the diagonal block: L[i][k0..k0+nb) = A[i][k0..) L_kk^-T
+__global__ __launch_bounds__(256) void trsm_panel_kernel(double* __restrict__ A, int n, int k0) {
+  __shared__ double T[kNB][kNB + 1];
+  const int nb = min(kNB, n - k0);
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    T[i][j] = (j <= i) ? A[(int64_t)(k0 + i) * n + (k0 + j)] : 0.0;
+  }
+  __syncthreads();
+  const int i = k0 + nb + blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double* row = A + (int64_t)i * n + k0;
+  double x[kNB];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j < nb) {
+      double s = row[j];
+      for (int t = 0; t < j; ++t) s -= x[t] * T[j][t];
+      x[j] = s / T[j][j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kNB; ++j)
+    if (j < nb) row[j] = x[j];
+}
+
+// trailing update of the lower triangle: A[i][j] -= sum_t L[i][k0+t] L[j][k0+t]
+// for i >= j >= k0 + nb, one 64 x 64 tile per workgroup (lower tiles only)
+__global__ __launch_bounds__(256) void syrk_update_kernel(double* __restrict__ A, int n, int k0) {
+  __shared__ double PI[kUT][kNB + 1];
+  __shared__ double PJ[kUT][kNB + 1];
+  const int nb = min(kNB, n - k0);
+  const int base = k0 + nb;
+  // tile (ti, tj), tj <= ti, from the linear block index
+  const int b = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= b) ++ti;
+  while (ti * (ti + 1) / 2 > b) --ti;
+  const int tj = b - ti * (ti + 1) / 2;
+  const int i0 = base + ti * kUT, j0 = base + tj * kUT;
+  for (int e = threadIdx.x; e < kUT * kNB; e += 256) {
+    const int r = e / kNB, t = e % kNB;
+    const int ii = i0 + r, jj = j0 + r;
+    PI[r][t] = (ii < n && t < nb) ? A[(int64_t)ii * n + k0 + t] : 0.0;
+    PJ[r][t] = (jj < n && t < nb) ? A[(int64_t)jj * n + k0 + t] : 0.0;
+  }
+  __syncthreads();
+  // thread: 4 rows (ty + 16 r) x 4 columns (tx + 16 c)
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
+#pragma unroll 2
+  for (int t = 0; t < kNB; ++t) {
+    double pi[4], pj[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pi[r] = PI[ty + 16 * r][t];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pj[c] = PJ[tx + 16 * c][t];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[r][c] += pi[r] * pj[c];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + ty + 16 * r;
+    if (i >= n) continue;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = j0 + tx + 16 * c;
+      if (j < n && j <= i) A[(int64_t)i * n + j] -= acc[r][c];
+    }
+  }
+}
+
+// forward substitution of the diagonal block [k0, k0+nb): x = L_kk^-1 x, one
+// wave, block and right-hand side staged in LDS (column-oriented)
+__global__ __launch_bounds__(64) void trsv_diag_lower_kernel(const double* __restrict__ A, int n, int k0,
+                                                             double* __restrict__ x) {
+  __shared__ double T[kNB][kNB + 1];
+  __shared__ double xs[kNB];
+  const int nb = min(kNB, n - k0);
+  for (int e = threadIdx.x; e < nb * nb; e += 64) T[e / nb][e % nb] = A[(int64_t)(k0 + e / nb) * n + (k0 + e % nb)];
+  if ((int)threadIdx.x < nb) xs[threadIdx.x] = x[k0 + threadIdx.x];
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    if (threadIdx.x == 0) xs[j] /= T[j][j];
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (i > j && i < nb) xs[i] -= T[i][j] * xs[j];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < nb) x[k0 + threadIdx.x] = xs[threadIdx.x];
+}
+
+// x[i] -= sum_t L[i][k0+t] x[k0+t] for i >= k0 + nb
+__global__ __launch_bounds__(256) void gemv_lower_kernel(const double* __restrict__ A, int n, int k0,
+                                                         double* __restrict__ x) {
+  __shared__ double xs[kNB];
+  const int nb = min(kNB, n - k0);
+  if (threadIdx.x < nb) xs[threadIdx.x] = x[k0 + threadIdx.x];
+  __syncthreads();
+  const int i = k0 + nb + blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double* row = A + (int64_t)i * n + k0;
+  double s = 0.0;
+  for (int t = 0; t < nb; ++t) s += row[t] * xs[t];
+  x[i] -= s;
+}
+
+// back substitution of the diagonal block: x = L_kk^-T x, one wave
+__global__ __launch_bounds__(64) void trsv_diag_upper_kernel(const double* __restrict__ A, int n, int k0,
+                                                             double* __restrict__ x) {
+  __shared__ double T[kNB][kNB + 1];
+  __shared__ double xs[kNB];
+  const int nb = min(kNB, n - k0);
+  for (int e = threadIdx.x; e < nb * nb; e += 64) T[e / nb][e % nb] = A[(int64_t)(k0 + e / nb) * n + (k0 + e % nb)];
+  if ((int)threadIdx.x < nb) xs[threadIdx.x] = x[k0 + threadIdx.x];
+  __syncthreads();
+  for (int j = nb - 1; j >= 0; --j) {
+    if (threadIdx.x == 0) xs[j] /= T[j][j];
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (i < j) xs[i] -= T[j][i] * xs[j];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < nb) x[k0 + threadIdx.x] = xs[threadIdx.x];
+}
+
+// x[i] -= sum_t L[k0+t][i] x[k0+t] for i < k0 (rows k0.. of L read along i: coalesced)
+__global__ __launch_bounds__(256) void gemv_upper_kernel(const double* __restrict__ A, int n, int k0,
+                                                         double* __restrict__ x) {
+  __shared__ double xs[kNB];
+  const int nb = min(kNB, n - k0);
+  if (threadIdx.x < nb) xs[threadIdx.x] = x[k0 + threadIdx.x];
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= k0) return;
+  double s = 0.0;
+  for (int t = 0; t < nb; ++t) s += A[(int64_t)(k0 + t) * n + i] * xs[t];
+  x[i] -= s;
+}
+
+__global__ void clear_status_kernel(int32_t* status) {
+  if (threadIdx.x == 0) status[0] = 0;
+}
+
+__global__ void copy_kernel(const double* __restrict__ a, int n, double* __restrict__ b) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = a[i];
+}
+
+int check_group(const apx_group* g) {
+  if (!g || !g->win || !g->pix_off || !g->col0 || !g->tile_ptr || !g->tile_src || (g->npair > 0 && !g->pair))
+    return set_error(APX_E_INVALID, "group: null array");
+  if (g->nx < 1 || g->ny < 1 || g->nsrc < 1 || g->np < 1)
+    return set_error(APX_E_INVALID, "group: empty image or no sources");
+  if (g->ntx != (g->nx + kTW - 1) / kTW || g->nty != (g->ny + kTH - 1) / kTH)
+    return set_error(APX_E_INVALID, "group: tile grid does not match the image");
+  if (g->sky_col >= g->np) return set_error(APX_E_INVALID, "group: sky column out of range");
+  return APX_OK;
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
+
+}  // namespace
+
+int group_image(const apx_group* g, const float* src, double sky, float* out, hipStream_t stream) {
+  if (int rc = check_group(g)) return rc;
+  if (!src || !out) return set_error(APX_E_INVALID, "group_image: null pointer");
+  hipLaunchKernelGGL(group_image_kernel, dim3((unsigned)(g->ntx * g->nty)), dim3(256), 0, stream, *g, src,
+                     sky, out);
+  return check_launch("group_image");
+}
+
+int64_t group_workspace_bytes(const apx_group* g) {
+  if (!g) return 0;
+  const int64_t npix = (int64_t)g->nx * g->ny;
+  return 8 * npix + 8 * 2 * 4096 + 256;
+}
+
+int group_normal(const apx_group* g, const float* J, const float* model, const float* data,
+                 const float* weight, const uint8_t* mask, double sky_dcol, double* H, double* grad,
+                 void* ws, int64_t ws_bytes, hipStream_t stream) {
+  if (int rc = check_group(g)) return rc;
+  if (!J || !model || !data || !H || !grad || !ws) return set_error(APX_E_INVALID, "group_normal: null pointer");
+  if (ws_bytes < group_workspace_bytes(g)) return set_error(APX_E_INVALID, "group_normal: workspace too small");
+  const int64_t np2 = (int64_t)g->np * g->np;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(np2)), dim3(256), 0, stream, H, np2, 0.0);
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(g->np)), dim3(256), 0, stream, grad, (int64_t)g->np, 0.0);
+  if (g->npair > 0)
+    hipLaunchKernelGGL(group_pair_kernel, dim3((unsigned)g->npair), dim3(256), 0, stream, *g, J, weight, mask, H);
+  hipLaunchKernelGGL(group_source_kernel, dim3((unsigned)g->nsrc), dim3(256), 0, stream, *g, J, model, data,
+                     weight, mask, sky_dcol, H, grad);
+  if (g->sky_col >= 0) {
+    const int64_t npix = (int64_t)g->nx * g->ny;
+    double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + 8 * npix);
+    const int nb = std::min(grid_for(npix), 512);
+    hipLaunchKernelGGL(group_image_sums_kernel, dim3(nb), dim3(256), 0, stream, npix, 0, model, data, weight,
+                       mask, (const double*)nullptr, part);
+    hipLaunchKernelGGL(group_sky_final_kernel, dim3(1), dim3(64), 0, stream, nb, 0, g->np, g->sky_col, sky_dcol,
+                       part, H, grad);
+  }
+  return check_launch("group_normal");
+}
+
+int group_rpp(const apx_group* g, const float* J, const float* m0, const float* m1, const float* weight,
+              const uint8_t* mask, const double* h, double d, double sky_dcol, double* out, void* ws,
+              int64_t ws_bytes, hipStream_t stream) {
+  if (int rc = check_group(g)) return rc;
+  if (!J || !m0 || !m1 || !h || !out || !ws) return set_error(APX_E_INVALID, "group_rpp: null pointer");
+  if (ws_bytes < group_workspace_bytes(g)) return set_error(APX_E_INVALID, "group_rpp: workspace too small");
+  const int64_t npix = (int64_t)g->nx * g->ny;
+  double* u = static_cast<double*>(ws);
+  double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + 8 * npix);
+  hipLaunchKernelGGL(group_rpp_weights_kernel, dim3((unsigned)(g->ntx * g->nty)), dim3(256), 0, stream, *g, J,
+                     m0, m1, weight, mask, h, d, sky_dcol, u);
+  hipLaunchKernelGGL(group_jtv_kernel, dim3((unsigned)g->nsrc), dim3(256), 0, stream, *g, J, (const double*)u,
+                     out);
+  if (g->sky_col >= 0) {
+    const int nb = std::min(grid_for(npix), 512);
+    hipLaunchKernelGGL(group_image_sums_kernel, dim3(nb), dim3(256), 0, stream, npix, 1, (const float*)nullptr,
+                       (const float*)nullptr, (const float*)nullptr, (const uint8_t*)nullptr, (const double*)u,
+                       part);
+    hipLaunchKernelGGL(group_sky_final_kernel, dim3(1), dim3(64), 0, stream, nb, 1, g->np, g->sky_col, sky_dcol,
+                       part, (double*)nullptr, out);
+  }
+  return check_launch("group_rpp");
+}
+
+int dense_damped_factor(int32_t n, const double* H, double L, double* A, int32_t* status, hipStream_t stream) {
+  if (!H || !A || !status) return set_error(APX_E_INVALID, "dense_factor: null pointer");
+  if (n < 1) return set_error(APX_E_INVALID, "dense_factor: n < 1");
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(damp_kernel, dim3(grid_for(nn)), dim3(256), 0, stream, H, n, L, A);
+  hipLaunchKernelGGL(clear_status_kernel, dim3(1), dim3(64), 0, stream, status);
+  for (int k0 = 0; k0 < n; k0 += kNB) {
+    const int nb = std::min(kNB, n - k0);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, stream, A, n, k0, status);
+    const int rest = n - k0 - nb;
+    if (rest <= 0) break;
+    hipLaunchKernelGGL(trsm_panel_kernel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, stream, A, n, k0);
+    const int T = (rest + kUT - 1) / kUT;
+    hipLaunchKernelGGL(syrk_update_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, stream, A, n, k0);
+  }
+  return check_launch("dense_factor");
+}
+
+int dense_cholesky_solve(int32_t n, const double* A, const double* b, double* x, hipStream_t stream) {
+  if (!A || !b || !x) return set_error(APX_E_INVALID, "dense_solve: null pointer");
+  if (n < 1) return set_error(APX_E_INVALID, "dense_solve: n < 1");
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, stream, b, n, x);
+  for (int k0 = 0; k0 < n; k0 += kNB) {
+    hipLaunchKernelGGL(trsv_diag_lower_kernel, dim3(1), dim3(64), 0, stream, A, n, k0, x);
+    const int rest = n - k0 - std::min(kNB, n - k0);
+    if (rest > 0)
+      hipLaunchKernelGGL(gemv_lower_kernel, dim3((unsigned)((rest + 255) / 256)), dim3(256), 0, stream, A, n, k0, x);
+  }
+  const int last = ((n - 1) / kNB) * kNB;
+  for (int k0 = last; k0 >= 0; k0 -= kNB) {
+    hipLaunchKernelGGL(trsv_diag_upper_kernel, dim3(1), dim3(64), 0, stream, A, n, k0, x);
+    if (k0 > 0)
+      hipLaunchKernelGGL(gemv_upper_kernel, dim3((unsigned)((k0 + 255) / 256)), dim3(256), 0, stream, A, n, k0, x);
+  }
+  return check_launch("dense_solve");
+}
+
+}  // namespace apx

@@ -108,6 +108,21 @@ class LM(BaseOptimizer):
         self._covariance_matrix = None
         dev = self.Y.device
         self.np = len(self.current_state)
+        # groups of Sersic galaxies on their own windows (+ a flat sky) beyond
+        # the dense kernels' 16 parameters go block-sparse (group_fit.py);
+        # sparse=True / False forces the choice
+        sparse = kwargs.get("sparse", None)
+        self._sparse = None
+        if sparse or (sparse is None and self.np > 16):
+            from .group_fit import SparseGroup
+            reason = SparseGroup.unsupported(self.model, self.fit_window)
+            if reason is None:
+                self._sparse = SparseGroup(self.model, self.fit_window)
+            elif sparse:
+                raise OptimizeStop(f"block-sparse fit not possible: {reason}")
+        if self._sparse is None and self.np > 16:
+            raise OptimizeStop(f"{self.np} parameters: the dense LM takes at most 16; the block-sparse one "
+                               f"takes groups of Sersic galaxies on windows plus a flat sky")
         self._ws_bytes = self.lib.apx_reduce_workspace_bytes(npix, max(self.np, 1))
         self._ws = torch.empty(self._ws_bytes, dtype=torch.uint8, device=dev)
         self._dbuf = torch.empty(1 + 2 * 64 + 64 * 64 + 64, dtype=torch.float64, device=dev)
@@ -118,11 +133,15 @@ class LM(BaseOptimizer):
     # -- device building blocks ---------------------------------------------
     def forward(self, parameters):
         self.n_forward += 1
+        if self._sparse is not None:
+            return self._sparse.forward(parameters)
         img = self.model(parameters=parameters.clone(), as_representation=True)
         return img.flatten("data").contiguous()
 
     def jacobian(self, parameters):
         self.n_jacobian += 1
+        if self._sparse is not None:
+            return self._sparse.jacobian(parameters)
         return self.model.jacobian(parameters=parameters.clone(), as_representation=True).flatten("data").contiguous()
 
     def _chi2(self, Ypred) -> torch.Tensor:
@@ -133,6 +152,8 @@ class LM(BaseOptimizer):
         return out[0] / self.ndf
 
     def _normal_equations(self, J, Y0):
+        if self._sparse is not None:
+            return self._sparse.normal_equations(J, Y0, self.Y, self.W, self.mask)
         n = self.np
         hess = torch.empty((n, n), dtype=torch.float64, device=self.Y.device)
         grad = torch.empty(n, dtype=torch.float64, device=self.Y.device)
@@ -143,12 +164,16 @@ class LM(BaseOptimizer):
 
     def _h(self, Lval, grad, hess) -> torch.Tensor:
         """lm.py:359-371: float64 damped solve on the device."""
+        if self._sparse is not None:
+            return self._sparse.solve(Lval, grad, hess)
         h = torch.empty_like(grad)
         L.check(self.lib.apx_lm_solve(L.ptr(hess), L.ptr(grad), self.np, float(Lval), L.ptr(h),
                                       L.ptr(self._status), L.stream()))
         return h
 
     def _rpp(self, J, Y0, Y1, h, d):
+        if self._sparse is not None:
+            return self._sparse.rpp(J, Y0, Y1, self.W, self.mask, h, d)
         out = torch.empty(self.np, dtype=torch.float64, device=self.Y.device)
         L.check(self.lib.apx_rpp(L.ptr(J), L.ptr(Y0), L.ptr(Y1), L.ptr(self.W), L.ptr(self.mask),
                                  self.Y.numel(), self.np, L.ptr(h), float(d), L.ptr(out), L.ptr(self._ws),
@@ -273,6 +298,18 @@ class LM(BaseOptimizer):
     @torch.no_grad()
     def update_hess_grad(self, natural=False):
         """lm.py:408-425."""
+        if self._sparse is not None:
+            # J with respect to values = J_rep / (dvalue / drep), column by column
+            J = self.jacobian(self.current_state)
+            Y = self.forward(self.current_state)
+            hess, grad = self._normal_equations(J, Y)
+            if natural:
+                d = torch.as_tensor(self.model.parameters.vector_dval_drep(self.current_state),
+                                    dtype=torch.float64, device=hess.device).reshape(-1)
+                hess = hess / d[:, None] / d[None, :]
+                grad = grad / d
+            self.hess, self.grad = hess, grad
+            return
         if natural:
             vals = self.model.parameters.vector_transform_rep_to_val(self.current_state)
             J = self.model.jacobian(parameters=vals, as_representation=False).flatten("data").contiguous()

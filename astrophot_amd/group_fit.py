@@ -1,0 +1,293 @@
+"""Block-sparse Levenberg-Marquardt pieces for a group of Sersic galaxies on
+windows of one image plus a flat sky (csrc/group.hip).
+
+``fit.LM`` on such a group (group_model_object.py:23-365 with Sersic
+components on their own windows, fit/lm.py:248-493) needs the group image,
+J^T W J, J^T W r, rpp and damped solves.  The dense route of the reference
+forms the (pixels x parameters) Jacobian, 117 GB for 500 sources on a
+2048 x 2048 image.  Here each source is one cutout of a batch (its window),
+rendered and differentiated by the batched kernels of ``csrc/batch.hip``;
+the group-level products come from the per-window blocks:
+
+* forward: each source rendered on its window; the integration threshold
+  divides the Sersic flux by the whole group window's working-grid pixel
+  count, as the reference's group sample does (every component is sampled on
+  the group window and kept on its own, model_object.py:258-375), which on
+  the kept pixels gives the same image; the image sums the windows and the
+  sky in group order;
+* Jacobian: each source on its own window with its own threshold, as the
+  reference's group Jacobian renders each component on its window
+  (_model_methods.py:260-347);
+* J^T W J: a 7 x 7 block per pair of overlapping windows plus the sky row;
+  rpp and J^T v per window; the damped solve is a blocked float64 Cholesky.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import AP_config
+from . import _lib as L
+from .batch import NP, BatchFit, Cutout
+from .errors import SpecificationConflict
+
+TILE_W, TILE_H = 64, 16
+SERSIC_ORDER = ("center", "q", "PA", "n", "Re", "Ie")
+
+
+def _same(a, b):
+    return abs(a - b) <= 1e-12 * max(1.0, abs(a), abs(b))
+
+
+def plan_tiles(wins: np.ndarray, nx: int, ny: int):
+    """Host plan of a group image: the sources whose window (x0, y0, w, h)
+    meets each 64 x 16 tile, ascending (CSR: tile_ptr, tile_src), and every
+    pair a <= b of overlapping windows, (a, a) included (flat, 2 per pair)."""
+    ntx, nty = (nx + TILE_W - 1) // TILE_W, (ny + TILE_H - 1) // TILE_H
+    W = np.asarray(wins, dtype=np.int64).reshape(-1, 4)
+    counts = np.zeros(ntx * nty, dtype=np.int64)
+    entries = []
+    for s, (x0, y0, w, h) in enumerate(W):
+        if w <= 0 or h <= 0:
+            continue
+        tys = np.arange(y0 // TILE_H, (y0 + h - 1) // TILE_H + 1)
+        txs = np.arange(x0 // TILE_W, (x0 + w - 1) // TILE_W + 1)
+        t = (tys[:, None] * ntx + txs[None, :]).reshape(-1)
+        entries.append((t, s))
+        counts[t] += 1
+    tile_ptr = np.zeros(ntx * nty + 1, dtype=np.int64)
+    tile_ptr[1:] = np.cumsum(counts)
+    tile_src = np.zeros(max(int(tile_ptr[-1]), 1), dtype=np.int32)
+    fill = tile_ptr[:-1].copy()
+    for t, s in entries:  # sources in ascending order, so each tile's list is ascending
+        tile_src[fill[t]] = s
+        fill[t] += 1
+    x1, y1 = W[:, 0] + W[:, 2], W[:, 1] + W[:, 3]
+    ov = ((np.minimum(x1[:, None], x1[None, :]) > np.maximum(W[:, 0][:, None], W[:, 0][None, :]))
+          & (np.minimum(y1[:, None], y1[None, :]) > np.maximum(W[:, 1][:, None], W[:, 1][None, :])))
+    a, b = np.nonzero(np.triu(ov))
+    pairs = np.stack([a, b], axis=1).astype(np.int32).reshape(-1)
+    return ntx, nty, tile_ptr.astype(np.int32), tile_src, pairs
+
+
+def scene_group(scene, target, values=None, name="scene"):
+    """The group model of a GroupScene: one Sersic galaxy per source on its
+    window (psf_mode "full" with the target's PSF) followed by a flat sky.
+    ``values`` (n, 7) replaces the scene's truth as the parameter values."""
+    from .models import AstroPhot_Model
+    v = scene.truth if values is None else np.asarray(values)
+    models = []
+    for k, (x0, x1, y0, y1) in enumerate(scene.windows):
+        cx, cy, q, pa, n, re, ie = (float(t) for t in v[k])
+        models.append(AstroPhot_Model(name=f"src{k}", model_type="sersic galaxy model", target=target,
+                                      psf_mode="full", window=[[int(x0), int(x1)], [int(y0), int(y1)]],
+                                      parameters={"center": [cx, cy], "q": q, "PA": pa, "n": n, "Re": re,
+                                                  "Ie": ie}))
+    models.append(AstroPhot_Model(name="sky", model_type="flat sky model", target=target,
+                                  parameters={"F": float(scene.sky_F)}))
+    return AstroPhot_Model(name=name, model_type="group model", models=models, target=target)
+
+
+class SparseGroup:
+    """A group held as per-window device blocks; see the module docstring."""
+
+    @staticmethod
+    def unsupported(model, fit_window) -> Optional[str]:
+        """Why this group cannot take the block-sparse route (None if it can)."""
+        from .image import PSF_Image
+        from .models import Flat_Sky, Group_Model, Sersic_Galaxy
+        if not isinstance(model, Group_Model):
+            return "not a group model"
+        sources, skies = [], []
+        for m in model.models.values():
+            if m.locked:
+                continue
+            if type(m) is Sersic_Galaxy:
+                sources.append(m)
+            elif isinstance(m, Flat_Sky):
+                skies.append(m)
+            else:
+                return f"{m.name}: only Sersic galaxies and a flat sky take the block-sparse route"
+        if not sources or len(skies) > 1:
+            return "needs at least one Sersic galaxy and at most one flat sky"
+        s0 = sources[0]
+        knobs = ("sampling_tolerance", "integrate_quad_level", "integrate_gridding", "integrate_max_depth",
+                 "softening")
+        P = fit_window._P
+        for m in sources:
+            if "full" not in m.psf_mode or m.psf_subpixel_shift != "bilinear":
+                return f"{m.name}: needs psf_mode 'full' with the bilinear sub-pixel shift"
+            if m.sampling_mode != "midpoint" or m.integrate_mode != "threshold":
+                return f"{m.name}: needs midpoint sampling with threshold integration"
+            if m.mask is not None or not isinstance(m.psf, PSF_Image):
+                return f"{m.name}: needs a PSF image and no model mask"
+            if any(getattr(m, k) != getattr(s0, k) for k in knobs):
+                return f"{m.name}: integration settings differ between sources"
+            if any(not _same(a, b) for a, b in zip(m.window._P, P)) or \
+                    any(not _same(a, b) for a, b in zip(m.psf.window._P, P)):
+                return f"{m.name}: pixel scale differs from the fit window's"
+            if len(m.parameters.vector_identities()) != NP or list(m.parameters.nodes) != list(SERSIC_ORDER):
+                return f"{m.name}: every Sersic parameter must be free"
+            w = m.window & fit_window
+            if (w.nx, w.ny) != (m.window.nx, m.window.ny):
+                return f"{m.name}: window not inside the fit window"
+            o = fit_window._x2p(*m.window._origin())
+            if abs(o[0] + 0.5 - round(o[0] + 0.5)) > 1e-9 or abs(o[1] + 0.5 - round(o[1] + 0.5)) > 1e-9:
+                return f"{m.name}: window not on the fit window's pixel grid"
+        for m in skies:
+            w = m.window & fit_window
+            if (w.nx, w.ny) != (fit_window.nx, fit_window.ny):
+                return "the sky must cover the fit window"
+            if m.mask is not None:
+                return "the sky must have no model mask"
+        if (model.window.nx, model.window.ny) != (fit_window.nx, fit_window.ny):
+            return "the fit window must be the group window"
+        return None
+
+    def __init__(self, model, fit_window):
+        reason = self.unsupported(model, fit_window)
+        if reason:
+            raise SpecificationConflict(f"block-sparse group fit: {reason}")
+        from .models import Flat_Sky
+        self.lib = L.load()
+        self.model = model
+        dev = AP_config.ap_device
+        self.nx, self.ny = fit_window.nx, fit_window.ny
+        ids = [str(i) for i in model.parameters.vector_identities()]
+        self.np = len(ids)
+        index = {pid: k for k, pid in enumerate(ids)}
+        members = [m for m in model.models.values() if not m.locked]
+        self.sources = [m for m in members if not isinstance(m, Flat_Sky)]
+        skies = [m for m in members if isinstance(m, Flat_Sky)]
+        self.sky = skies[0] if skies else None
+        col0 = []
+        for m in self.sources:
+            mids = [str(i) for i in m.parameters.vector_identities()]
+            c = index[mids[0]]
+            if [index[i] for i in mids] != list(range(c, c + NP)):
+                raise SpecificationConflict("block-sparse group fit: a source's parameters are not contiguous")
+            col0.append(c)
+        self.sky_col = -1
+        if self.sky is not None and len(self.sky.parameters.vector_identities()) == 1:
+            self.sky_col = index[str(self.sky.parameters.vector_identities()[0])]
+        self.sky_after = -1
+        if self.sky is not None:
+            self.sky_after = sum(1 for m in members[:members.index(self.sky)] if m in self.sources)
+        # windows on the fit window's pixel grid
+        wins = []
+        for m in self.sources:
+            rows, cols = fit_window.get_self_indices(m.window)
+            wins.append((cols.start, rows.start, cols.stop - cols.start, rows.stop - rows.start))
+        self.wins = np.array(wins, dtype=np.int32)
+        s0 = self.sources[0]
+        cutouts = [Cutout(np.broadcast_to(0.0, (h, w)), None, m.psf.data.detach().cpu().numpy(),
+                          origin=m.window._origin())
+                   for m, (x0, y0, w, h) in zip(self.sources, wins)]
+        npix = int(sum(int(w) * int(h) for _, _, w, h in wins))
+        zeros = torch.zeros(npix, dtype=torch.float32, device=dev)
+        self.bf = BatchFit(cutouts, pixelscale=list(fit_window._P), sampling_tolerance=s0.sampling_tolerance,
+                           softening=s0.softening, integrate_quad_level=s0.integrate_quad_level,
+                           integrate_gridding=s0.integrate_gridding, integrate_max_depth=s0.integrate_max_depth,
+                           data=zeros, variance=torch.ones_like(zeros))
+        bf = self.bf
+        # two descriptor copies: sample threshold over the whole group window, Jacobian over each window
+        gal_full = (type(bf.gal_host[0]) * len(self.sources))()
+        ctypes.memmove(gal_full, bf.gal_host, ctypes.sizeof(gal_full))
+        for k, m in enumerate(self.sources):
+            bx, by = m.psf.psf_border_int
+            gal_full[k].ref_npix = (self.nx + 2 * bx) * (self.ny + 2 * by)
+        self.gal_jac = bf.gal_dev
+        self.gal_full = torch.frombuffer(bytearray(gal_full), dtype=torch.uint8).to(dev)
+        pix_off = np.array([bf.gal_host[k].pix_off for k in range(len(self.sources))], dtype=np.int64)
+        ntx, nty, tile_ptr, tile_src, pairs = plan_tiles(self.wins, self.nx, self.ny)
+        self.npair = len(pairs) // 2
+        to = lambda x, dt: torch.as_tensor(x, dtype=dt, device=dev).contiguous()
+        self._arrays = dict(win=to(self.wins.reshape(-1), torch.int32), pix_off=to(pix_off, torch.int64),
+                            col0=to(np.array(col0, dtype=np.int32), torch.int32),
+                            tile_ptr=to(tile_ptr, torch.int32), tile_src=to(tile_src, torch.int32),
+                            pair=to(pairs if len(pairs) else np.zeros(2, np.int32), torch.int32))
+        g = L.Group()
+        g.nx, g.ny, g.nsrc, g.np = self.nx, self.ny, len(self.sources), self.np
+        g.sky_col, g.sky_after = self.sky_col, self.sky_after
+        g.ntx, g.nty, g.npair = ntx, nty, self.npair
+        for k, t in self._arrays.items():
+            setattr(g, k, t.data_ptr())
+        g.nsrcpix = bf.npix
+        self.g = g
+        self.col_index = to(np.array(col0, dtype=np.int64)[:, None] + np.arange(NP)[None, :], torch.int64)
+        self.src_model = torch.zeros(bf.npix, dtype=torch.float32, device=dev)
+        self.wsb = self.lib.apx_group_workspace_bytes(ctypes.byref(g))
+        self.ws = torch.zeros(self.wsb, dtype=torch.uint8, device=dev)
+        self.factor = torch.zeros((self.np, self.np), dtype=torch.float64, device=dev)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._factor_key = None
+        self._sky_dcol = 0.0
+        self.area = fit_window._area
+
+    # -- state -> per-source representation, sky value ---------------------
+    def _sources_rep(self, state):
+        st = torch.as_tensor(state, dtype=torch.float64).to(self.src_model.device)
+        return st[self.col_index].contiguous()
+
+    def _sky_F(self, state):
+        if self.sky is None:
+            return None
+        if self.sky_col >= 0:
+            return float(torch.as_tensor(state)[self.sky_col])
+        return float(self.sky["F"].value)
+
+    # -- LM pieces ------------------------------------------------------------
+    def forward(self, state) -> torch.Tensor:
+        """The group image (ny x nx float32, flattened) at ``state`` (representation)."""
+        bf = self.bf
+        bf.b.gal = self.gal_full.data_ptr()
+        try:
+            bf.render(self._sources_rep(state), out=self.src_model)
+        finally:
+            bf.b.gal = self.gal_jac.data_ptr()
+        F = self._sky_F(state)
+        sky = 0.0 if F is None else self.area * 10 ** F
+        out = torch.empty(self.nx * self.ny, dtype=torch.float32, device=self.src_model.device)
+        L.check(self.lib.apx_group_image(ctypes.byref(self.g), L.ptr(self.src_model), float(sky), L.ptr(out),
+                                         L.stream()))
+        return out
+
+    def jacobian(self, state) -> torch.Tensor:
+        """Per-window Jacobian blocks (7 x batch pixels, parameter-major) at ``state``."""
+        self.bf.jacobian(self._sources_rep(state), out=self.src_model)
+        F = self._sky_F(state)
+        self._sky_dcol = 0.0 if F is None else self.area * 10 ** F * math.log(10)
+        return self.bf.jac
+
+    def normal_equations(self, J, model, data, weight, mask):
+        dev = data.device
+        hess = torch.empty((self.np, self.np), dtype=torch.float64, device=dev)
+        grad = torch.empty(self.np, dtype=torch.float64, device=dev)
+        L.check(self.lib.apx_group_normal(ctypes.byref(self.g), L.ptr(J), L.ptr(model), L.ptr(data),
+                                          L.ptr(weight), L.ptr(mask), self._sky_dcol, L.ptr(hess), L.ptr(grad),
+                                          L.ptr(self.ws), self.wsb, L.stream()))
+        return hess, grad
+
+    def rpp(self, J, m0, m1, weight, mask, h, d):
+        out = torch.empty(self.np, dtype=torch.float64, device=m0.device)
+        L.check(self.lib.apx_group_rpp(ctypes.byref(self.g), L.ptr(J), L.ptr(m0), L.ptr(m1), L.ptr(weight),
+                                       L.ptr(mask), L.ptr(h), float(d), self._sky_dcol, L.ptr(out),
+                                       L.ptr(self.ws), self.wsb, L.stream()))
+        return out
+
+    def solve(self, Lval, vec, hess) -> torch.Tensor:
+        """h = A(L)^-1 vec (fit/lm.py:359-371); the factor of A(L) is kept
+        while L and the normal matrix stay the same."""
+        if self._factor_key is None or self._factor_key[0] != float(Lval) or self._factor_key[1] is not hess:
+            L.check(self.lib.apx_dense_damped_factor(self.np, L.ptr(hess), float(Lval), L.ptr(self.factor),
+                                                     L.ptr(self.status), L.stream()))
+            self._factor_key = (float(Lval), hess)  # holds hess, so its memory is not reused meanwhile
+        x = torch.empty_like(vec)
+        L.check(self.lib.apx_dense_cholesky_solve(self.np, L.ptr(self.factor), L.ptr(vec.contiguous()), L.ptr(x),
+                                                  L.stream()))
+        return x

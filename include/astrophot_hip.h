@@ -240,6 +240,10 @@ typedef struct apx_batch_galaxy {
   int64_t chunk_off;  /* first 256-pixel working chunk (global index) */
   int64_t tile_off;   /* first 64 x 8 output tile (global index) */
   double origin[2];   /* plane coordinates of the cutout corner (pixel -0.5,-0.5) */
+  int64_t ref_npix;   /* pixel count the Sersic flux is divided by in the integration
+                         threshold (sersic_model.py:87-89); 0 = this cutout's working
+                         grid.  A source sampled as part of a larger window (a group,
+                         group_model_object.py:183-231) sets that window's working grid */
 } apx_batch_galaxy;
 
 typedef struct apx_batch {
@@ -335,6 +339,71 @@ int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t
 
 /* Last error message of the calling thread. */
 const char* apx_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * A group model whose sources sit on windows of one image (north-star
+ * configuration C3: hundreds of Sersic sources plus a flat sky fitted
+ * jointly).  Each source is one cutout of an apx_batch (its window on the
+ * image), so its model and its 7 Jacobian columns (parameter-major, as
+ * apx_batch_jacobian writes them) exist only on that window.  These entry
+ * points give Levenberg-Marquardt (fit/lm.py:248-406) what it needs from the
+ * whole group without forming the dense pixels x parameters Jacobian of
+ * group_model_object.py:233-283.  The host builds the per-tile source lists
+ * (64 x 16 image tiles) and the list of overlapping window pairs.
+ * ------------------------------------------------------------------------- */
+typedef struct apx_group {
+  int32_t nx, ny;           /* image (the fit window), row-major */
+  int32_t nsrc;             /* sources = cutouts of the batch, in group order */
+  int32_t np;               /* group parameters: 7 per source (+ 1 sky) */
+  int32_t sky_col;          /* column of the sky's F (representation), -1 = no sky */
+  int32_t sky_after;        /* sources ahead of the sky in the group order; -1 = no sky */
+  int32_t ntx, nty;         /* (nx + 63) / 64, (ny + 15) / 16 */
+  int32_t npair;            /* overlapping window pairs, (a, a) included */
+  int32_t reserved;
+  const int32_t* win;       /* nsrc x 4: x0, y0, width, height on the image (inside it) */
+  const int64_t* pix_off;   /* nsrc: the window's first pixel in per-source buffers */
+  const int32_t* col0;      /* nsrc: the source's first column (7 consecutive) */
+  const int32_t* tile_ptr;  /* ntx * nty + 1: CSR offsets into tile_src */
+  const int32_t* tile_src;  /* sources whose window meets each tile, ascending */
+  const int32_t* pair;      /* npair x 2: (a, b), a <= b, windows overlapping */
+  int64_t nsrcpix;          /* per-source buffer length (the batch's npix) */
+} apx_group;
+
+/* Workspace (bytes) apx_group_normal and apx_group_rpp need. */
+int64_t apx_group_workspace_bytes(const apx_group* group);
+
+/* out (ny x nx float32) = the group image: every pixel sums the sources whose
+ * window covers it, in group order, with the flat sky value `sky` added at
+ * its place in that order (group_model_object.py:183-231). */
+int apx_group_image(const apx_group* group, const float* src_model, double sky, float* out,
+                    void* stream);
+
+/* H (np x np float64, row-major) = J^T W J and grad (np) = J^T W (data -
+ * model) over unmasked pixels (fit/lm.py:387-399): a 7 x 7 block per
+ * overlapping pair, each source's sky-column entries, and the sky's own,
+ * where the sky column of J is the constant sky_dcol = d(sky value)/d(rep of F).
+ * J: 7 x nsrcpix float32 (parameter-major per-source blocks). */
+int apx_group_normal(const apx_group* group, const float* J, const float* model, const float* data,
+                     const float* weight, const uint8_t* mask, double sky_dcol, double* hess,
+                     double* grad, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* rpp (np, float64) = J^T ((2/d) (W (m1 - m0) / d - W (J h))) over unmasked
+ * pixels (fit/lm.py:401-406). */
+int apx_group_rpp(const apx_group* group, const float* J, const float* m0, const float* m1,
+                  const float* weight, const uint8_t* mask, const double* h, double d, double sky_dcol,
+                  double* rpp, void* workspace, int64_t workspace_bytes, void* stream);
+
+/* Damped matrix of fit/lm.py:359-371, A(L)_ii = H_ii + L (1 + H_ii), A(L)_ij =
+ * H_ij / (1 + L), factored A = L L^T in place into `factor` (n x n float64,
+ * lower triangle; blocked right-looking Cholesky).  status[0] (device) = 0, or
+ * k + 1 when pivot k was not positive (the factor then holds NaN and so will
+ * every solution, which the caller's LM treats as a failed step). */
+int apx_dense_damped_factor(int32_t n, const double* hess, double L, double* factor, int32_t* status,
+                            void* stream);
+
+/* x = A^-1 b from the factor of apx_dense_damped_factor (two blocked
+ * triangular solves; float64, device). */
+int apx_dense_cholesky_solve(int32_t n, const double* factor, const double* b, double* x, void* stream);
 
 /* -------------------------------------------------------------------------
  * Learned initialiser: primitives of a small CNN (4 convolutions 3x3 /

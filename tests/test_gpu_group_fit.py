@@ -1,0 +1,180 @@
+"""The block-sparse group fit (csrc/group.hip, astrophot_amd/group_fit.py)
+against the dense group path and the float64 oracle.
+
+Scene: 6 Sersic sources on their own windows of a 120 x 120 image plus a
+flat sky (43 parameters), Moffat PSF 11 x 11, variance 1, seeded noise.
+
+* image: the block-sparse forward vs the dense group render and the oracle
+  (1e-5 of the max);
+* J^T W J, J^T W r and rpp vs the same products of the dense group Jacobian
+  (float32 GPU, 1e-5 of sqrt(H_ii H_jj)) and of the oracle's forward-mode
+  Jacobian (5e-4);
+* the blocked float64 damped Cholesky solve vs numpy (1e-9);
+* LM from a +-10 % start vs the oracle's LM on the same data, sources at
+  S/N 100-1000: parameters 1e-4, chi^2 1e-6, both "success".  With faint
+  sources (S/N down to 10) the two LMs follow the same damping sequence for
+  the first dozen steps, then stop at different points of flat chi^2 valleys
+  (LM stops on a 1e-5 relative chi^2 change); there the GPU fit's chi^2,
+  evaluated by the oracle, must be no worse than the oracle fit's (1e-6).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import astrophot_amd as ap
+from astrophot_amd import _lib as L
+from astrophot_amd.group_fit import SparseGroup, scene_group
+from astrophot_amd.population import group_scene, perturbed_start
+from oracle import hotpath as H
+
+pytestmark = pytest.mark.gpu
+
+SIDE = 120
+
+
+def oracle_group(s, values, F):
+    comps = []
+    psf = torch.tensor(s.psf)
+    for (x0, x1, y0, y1), v in zip(s.windows, values):
+        g = H.Grid(int(x1 - x0), int(y1 - y0), ref_ij=(-0.5 - x0, -0.5 - y0))
+        p = {"center": [v[0], v[1]], "q": v[2], "PA": v[3], "n": v[4], "Re": v[5], "Ie": v[6]}
+        comps.append(H.Component("sersic", g, p, psf=psf, psf_mode="full"))
+    comps.append(H.Component("flat_sky", H.Grid(SIDE, SIDE), {"F": F}))
+    return H.Model(comps, window=H.Grid(SIDE, SIDE))
+
+
+def make_scene(snr_range=(10.0, 1000.0)):
+    s = group_scene(6, 11, side=SIDE, psf_side=11, min_half=16, re_range=(1.5, 5.0), snr_range=snr_range)
+    om = oracle_group(s, s.truth, s.sky_F)
+    truth_img = om.forward(om.vector_representation()).numpy()
+    rng = np.random.default_rng(5)
+    data = truth_img + rng.normal(size=truth_img.shape) * s.sky_sigma
+    var = np.full_like(data, s.sky_sigma ** 2)
+    tgt = ap.image.Target_Image(data=data, pixelscale=1.0, variance=var,
+                                psf=ap.image.PSF_Image(data=s.psf, pixelscale=1.0))
+    return s, om, data, var, tgt
+
+
+@pytest.fixture(scope="module")
+def scene():
+    return make_scene()
+
+
+def test_dense_damped_cholesky_solve():
+    lib = L.load()
+    rng = np.random.default_rng(0)
+    for n in (37, 300, 1000):
+        A = rng.normal(size=(n, n + 5))
+        Hm = A @ A.T
+        b = rng.normal(size=n)
+        Hd = torch.tensor(Hm, device="cuda")
+        bd = torch.tensor(b, device="cuda")
+        fac = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        st = torch.zeros(1, dtype=torch.int32, device="cuda")
+        x = torch.empty(n, dtype=torch.float64, device="cuda")
+        for Lval in (0.0, 1e-3, 1.0):
+            L.check(lib.apx_dense_damped_factor(n, L.ptr(Hd), Lval, L.ptr(fac), L.ptr(st), L.stream()))
+            L.check(lib.apx_dense_cholesky_solve(n, L.ptr(fac), L.ptr(bd), L.ptr(x), L.stream()))
+            Ad = Hm / (1 + Lval)
+            np.fill_diagonal(Ad, np.diag(Hm) + Lval * (1 + np.diag(Hm)))
+            ref = np.linalg.solve(Ad, b)
+            assert int(st.item()) == 0
+            assert np.abs(x.cpu().numpy() - ref).max() <= 1e-9 * np.abs(ref).max(), (n, Lval)
+
+
+def test_forward_matches_dense_group_and_oracle(scene):
+    s, om, data, var, tgt = scene
+    g = scene_group(s, tgt)
+    sg = SparseGroup(g, g.window)
+    rep = g.parameters.vector_representation()
+    assert np.allclose(rep.numpy(), om.vector_representation().numpy(), rtol=1e-12, atol=1e-12)
+    img = sg.forward(rep).view(SIDE, SIDE)
+    dense = g().data
+    ref = om.forward(om.vector_representation())
+    scale = float(ref.abs().max())
+    assert float((img.double().cpu() - ref).abs().max()) < 1e-5 * scale
+    assert float((img - dense).abs().max()) < 1e-6 * scale
+
+
+def test_normal_equations_and_rpp(scene):
+    s, om, data, var, tgt = scene
+    start = perturbed_start(s.truth, 3, 0.05)
+    g = scene_group(s, tgt, values=start)
+    sg = SparseGroup(g, g.window)
+    rep = g.parameters.vector_representation().clone()
+    Y = torch.tensor(data, dtype=torch.float32, device="cuda").reshape(-1)
+    W = torch.ones_like(Y)
+    m0 = sg.forward(rep)
+    J = sg.jacobian(rep)
+    Hs, gs = sg.normal_equations(J, m0, Y, W, None)
+    Jd = g.jacobian(parameters=rep.clone(), as_representation=True).data.double().reshape(-1, len(rep))
+    Hd = Jd.T @ Jd
+    gd = Jd.T @ (Y.double() - m0.double())
+    norm = torch.sqrt(torch.diag(Hd))[:, None] * torch.sqrt(torch.diag(Hd))[None, :]
+    assert float(((Hs - Hd).abs() / norm).max()) < 1e-5
+    assert float((gs - gd).abs().max()) < 1e-5 * float(gd.abs().max())
+    # the oracle's forward-mode Jacobian
+    Jo = om.jacobian(rep.cpu()).reshape(-1, len(rep))
+    Ho = Jo.T @ Jo
+    norm_o = torch.sqrt(torch.diag(Ho))[:, None] * torch.sqrt(torch.diag(Ho))[None, :]
+    assert float(((Hs.cpu() - Ho).abs() / norm_o).max()) < 5e-4
+    # the same products from the block-sparse J laid out densely: the kernels'
+    # arithmetic alone (float64 sums of the same float32 terms)
+    Jb = torch.zeros((SIDE * SIDE, len(rep)), dtype=torch.float64, device="cuda")
+    Jk = J.view(7, -1)
+    for a, (x0, y0, w, h) in enumerate(sg.wins.tolist()):
+        off = int(sg.bf.gal_host[a].pix_off)
+        idx = ((torch.arange(h, device="cuda")[:, None] + y0) * SIDE + torch.arange(w, device="cuda")[None, :] + x0)
+        c = int(sg._arrays["col0"][a])
+        Jb[idx.reshape(-1), c:c + 7] = Jk[:, off:off + w * h].t().double()
+    Jb[:, sg.sky_col] = sg._sky_dcol
+    Hb = Jb.T @ Jb
+    assert float(((Hs - Hb).abs() / norm).max()) < 1e-10
+    assert float((gs - Jb.T @ (Y.double() - m0.double())).abs().max()) < 1e-10 * float(gd.abs().max())
+    # rpp against the dense formula (fit/lm.py:401-406), on the same J; on the
+    # dense group J it is a difference of nearly equal terms, hence 1e-2
+    h = torch.tensor(np.random.default_rng(2).normal(size=len(rep)) * 1e-3, dtype=torch.float64, device="cuda")
+    d = 0.1
+    m1 = sg.forward(rep + d * h.cpu())
+    rs = sg.rpp(J, m0, m1, W, None, h, d)
+    rb = Jb.T @ ((2 / d) * ((m1.double() - m0.double()) / d - Jb @ h))
+    assert float((rs - rb).abs().max()) < 1e-9 * float(rb.abs().max())
+    rd = Jd.T @ ((2 / d) * ((m1.double() - m0.double()) / d - Jd @ h))
+    assert float((rs - rd).abs().max()) < 1e-2 * float(rd.abs().max())
+
+
+@pytest.mark.parametrize("snr", ["bright", "faint"])
+def test_sparse_lm_matches_oracle_lm(snr):
+    s, om, data, var, tgt = make_scene((100.0, 1000.0) if snr == "bright" else (10.0, 1000.0))
+    start = perturbed_start(s.truth, 4, 0.1)
+    F0 = s.sky_F + 0.02
+    om0 = oracle_group(s, start, F0)
+    o_start = om0.vector_representation()
+    lm_o = H.LM(om0, torch.tensor(data), weight=torch.tensor(1 / var), initial_state=o_start).fit()
+    g = scene_group(s, tgt, values=start)
+    g.models["sky"]["F"].value = F0
+    g_start = g.parameters.vector_representation()
+    assert torch.allclose(g_start, o_start, rtol=1e-12, atol=1e-12)
+    lm_g = ap.fit.LM(g, initial_state=g_start).fit()
+    assert lm_g._sparse is not None
+    assert lm_o.message.startswith("success"), lm_o.message
+    assert lm_g.message.startswith("success"), lm_g.message
+    r_o = torch.as_tensor(lm_o.res())
+    if snr == "faint":
+        chi_oo = lm_o._chi2(om0.forward(r_o).reshape(-1)).item()
+        chi_og = lm_o._chi2(om0.forward(torch.as_tensor(lm_g.res())).reshape(-1)).item()
+        assert chi_og <= chi_oo * (1 + 1e-6), (chi_og, chi_oo)
+        return
+    vals = []
+    for c, r in zip(om0.components, om0.split(r_o)):
+        vv = c.rep_to_values(r)
+        vals.append(torch.cat([vv[name].reshape(-1) for name, *_ in c.free]))
+    v_o = torch.cat(vals)
+    v_g = g.parameters.vector_values()
+    err = ((v_g - v_o).abs() / v_o.abs().clamp_min(1.0)).max().item()
+    assert err < 1e-4, err
+    chi_o, chi_g = min(lm_o.loss_history), min(lm_g.loss_history)
+    assert abs(chi_g - chi_o) / chi_o < 1e-6, (chi_g, chi_o)

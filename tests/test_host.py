@@ -39,7 +39,7 @@ def test_struct_layouts_match_header():
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     from astrophot_amd import batch as B
     for cname, cls in (("apx_component", L.Component), ("apx_conv_job", L.ConvJob),
-                       ("apx_batch_galaxy", B.Galaxy), ("apx_batch", B.Batch)):
+                       ("apx_batch_galaxy", B.Galaxy), ("apx_batch", B.Batch), ("apx_group", L.Group)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), text, re.S).group(1)
         fields = []
         for decl in body.split(";"):

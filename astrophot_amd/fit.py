@@ -115,9 +115,9 @@ class LM(BaseOptimizer):
         self._sparse = None
         if sparse or (sparse is None and self.np > 16):
             from .group_fit import SparseGroup
-            reason = SparseGroup.unsupported(self.model, self.fit_window)
+            reason = SparseGroup.unsupported(self.model, tgt.window)
             if reason is None:
-                self._sparse = SparseGroup(self.model, self.fit_window)
+                self._sparse = SparseGroup(self.model, tgt.window)
             elif sparse:
                 raise OptimizeStop(f"block-sparse fit not possible: {reason}")
         if self._sparse is None and self.np > 16:

@@ -98,7 +98,11 @@ class SparseGroup:
 
     @staticmethod
     def unsupported(model, fit_window) -> Optional[str]:
-        """Why this group cannot take the block-sparse route (None if it can)."""
+        """Why this group cannot take the block-sparse route (None if it can).
+        ``fit_window`` is the image grid the fit runs on: the target's pixels
+        under the group window (a group window can reach past the image,
+        window_object.py:483-495, and the model image is the target's part,
+        core_model.py:193-204)."""
         from .image import PSF_Image
         from .models import Flat_Sky, Group_Model, Sersic_Galaxy
         if not isinstance(model, Group_Model):
@@ -145,7 +149,8 @@ class SparseGroup:
                 return "the sky must cover the fit window"
             if m.mask is not None:
                 return "the sky must have no model mask"
-        if (model.window.nx, model.window.ny) != (fit_window.nx, fit_window.ny):
+        gw = model.window & model.target.window if model.target is not None else model.window
+        if (gw.nx, gw.ny) != (fit_window.nx, fit_window.ny):
             return "the fit window must be the group window"
         return None
 

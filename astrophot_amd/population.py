@@ -60,11 +60,15 @@ def sample_population(n: int, seed: int, side_range=(32, 256), re_range=(2.0, 40
     return Population(truth, sides, sig, psfs, sky, snr)
 
 
-def perturbed_start(truth: np.ndarray, seed: int, frac: float = 0.1) -> np.ndarray:
+def perturbed_start(truth: np.ndarray, seed: int, frac: float = 0.1, center_px=None) -> np.ndarray:
     """Every parameter times (1 + U(-frac, frac)), kept inside its limits
-    (the C0 start rule)."""
+    (the C0 start rule).  With ``center_px`` the centres move by
+    U(-center_px, center_px) pixels instead, which keeps sources far from the
+    plane origin inside their windows."""
     rng = np.random.default_rng(seed)
     s = truth * (1 + rng.uniform(-frac, frac, truth.shape))
+    if center_px is not None:
+        s[:, :2] = truth[:, :2] + rng.uniform(-center_px, center_px, (truth.shape[0], 2))
     s[:, 2] = np.clip(s[:, 2], 0.02, 0.999)
     s[:, 4] = np.clip(s[:, 4], 0.37, 7.99)
     s[:, 5] = np.maximum(s[:, 5], 0.05)

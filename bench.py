@@ -13,10 +13,12 @@ LM iterations saved); the plain single-galaxy GPU path (astrophot_amd.fit.LM,
 one cutout at a time, from the moment start) on a subset with per-fit
 agreement; the dominant kernel's roofline measured live with HIP events; the
 float64 oracle LM on the host cores as the CPU baseline (same cutouts, same
-start); the 4096-cutout 32-256 px population from a perturbed truth; and the
-512x512 bulge+disk render (render Mpix/s, Jacobian evals/s).
+start); the 4096-cutout 32-256 px population from a perturbed truth; the
+512x512 bulge+disk render (render Mpix/s, Jacobian evals/s); and the joint
+fit of 500 sources plus sky on one 2048x2048 image (3501 parameters,
+block-sparse LM).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ngal4 65536] [--no-cpu] [--no-c1] [--no-c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ngal4 65536] [--no-cpu] [--no-c1] [--no-c2] [--no-c3]
 """
 
 from __future__ import annotations
@@ -342,6 +344,59 @@ def c2_leg(args):
     return out, (pop, data, var, start)
 
 
+def c3_leg(nsrc=500, side=2048, seed=SEED + 3):
+    """Joint fit of one image: nsrc Sersic sources on their own windows (half
+    side max(5 Re, 33) px) plus a flat sky, one shared 65 x 65 Moffat PSF,
+    noise sigma 1, from the truth with centres moved U(-1, 1) px and every other
+    parameter scaled by 1 + U(-0.1, 0.1); block-sparse LM (fit.LM)."""
+    import astrophot_amd as ap
+    from astrophot_amd.group_fit import SparseGroup, scene_group
+    from astrophot_amd.population import group_scene, perturbed_start
+
+    s = group_scene(nsrc, seed, side=side, psf_side=65)
+    psf = ap.image.PSF_Image(data=s.psf, pixelscale=1.0)
+    ones = np.ones((side, side), dtype=np.float32)
+    g0 = scene_group(s, ap.image.Target_Image(data=0 * ones, pixelscale=1.0, variance=ones, psf=psf))
+    sg = SparseGroup(g0, g0.target[g0.window].window)
+    truth = g0.parameters.vector_representation()
+    img = sg.forward(truth)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    data = (img + s.sky_sigma * torch.randn(img.shape, generator=gen, device="cuda")).view(side, side)
+    # per-stage device times on the truth
+    Y, W = data.reshape(-1).contiguous(), torch.ones(side * side, dtype=torch.float32, device="cuda")
+    J = sg.jacobian(truth)
+    Hs, gs = sg.normal_equations(J, img, Y, W, None)
+    stages = {"forward_ms": timed_events(lambda: sg.forward(truth), n=6) / 1e3,
+              "jacobian_ms": timed_events(lambda: sg.jacobian(truth), n=4) / 1e3,
+              "normal_equations_ms": timed_events(lambda: sg.normal_equations(J, img, Y, W, None), n=6) / 1e3,
+              "factor_and_solve_ms": timed_events(lambda: (setattr(sg, "_factor_key", None),
+                                                           sg.solve(1.0, gs, Hs)), n=4) / 1e3,
+              "solve_ms": timed_events(lambda: sg.solve(1.0, gs, Hs), n=6) / 1e3,
+              "rpp_ms": timed_events(lambda: sg.rpp(J, img, img, W, None, gs, 0.1), n=6) / 1e3}
+    start = perturbed_start(s.truth, seed + 1, 0.1, center_px=1.0)
+    tgt = ap.image.Target_Image(data=data.cpu().numpy(), pixelscale=1.0, variance=ones, psf=psf)
+    out = {}
+    for label in ("warmup", "timed"):
+        g = scene_group(s, tgt, values=start)
+        g.models["sky"]["F"].value = s.sky_F + 0.02
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lm = ap.fit.LM(g, initial_state=g.parameters.vector_representation()).fit()
+        torch.cuda.synchronize()
+        out[label] = (time.perf_counter() - t0, lm)
+    sec, lm = out["timed"]
+    return {
+        "workload": f"{nsrc} Sersic sources on windows of one {side}x{side} image + flat sky, shared 65x65 Moffat "
+                    f"PSF (beta {s.beta:.3f}), joint LM of {int(truth.numel())} parameters",
+        "seed": seed, "parameters": int(truth.numel()), "window_pixels": int(sg.bf.npix), "pairs": sg.npair,
+        "seconds_per_fit": sec, "iterations": len(lm.loss_history) - 1, "message": lm.message,
+        "chi2_start": lm.loss_history[0], "chi2_end": min(lm.loss_history),
+        "forwards": lm.n_forward, "jacobians": lm.n_jacobian, "stages": stages,
+        "dense_jacobian_gb": side * side * int(truth.numel()) * 4 / 1e9,
+    }
+
+
 def kernel_only(args):
     """For counter collection (rocprofv3 --pmc): the headline batch's
     Jacobian-stage convolution over every cutout 3 times, then a plain 1 GiB
@@ -377,6 +432,7 @@ def main():
     ap_.add_argument("--no-cpu", action="store_true")
     ap_.add_argument("--no-c1", action="store_true")
     ap_.add_argument("--no-c2", action="store_true")
+    ap_.add_argument("--no-c3", action="store_true")
     ap_.add_argument("--kernel-only", action="store_true",
                      help="run only the roofline kernel (and a calibration copy) for counter collection")
     args = ap_.parse_args()
@@ -404,6 +460,8 @@ def main():
         line["c2"], _ = c2_leg(args)
     if not args.no_c1:
         line["c1"] = c1_render_leg()
+    if not args.no_c3:
+        line["c3"] = c3_leg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(*c4_data)
         line["gpu_vs_cpu"] = line["value"] / line["cpu_baseline"]["value"]

@@ -88,7 +88,7 @@ def test_dense_damped_cholesky_solve():
 def test_forward_matches_dense_group_and_oracle(scene):
     s, om, data, var, tgt = scene
     g = scene_group(s, tgt)
-    sg = SparseGroup(g, g.window)
+    sg = SparseGroup(g, g.target[g.window].window)
     rep = g.parameters.vector_representation()
     assert np.allclose(rep.numpy(), om.vector_representation().numpy(), rtol=1e-12, atol=1e-12)
     img = sg.forward(rep).view(SIDE, SIDE)
@@ -103,7 +103,7 @@ def test_normal_equations_and_rpp(scene):
     s, om, data, var, tgt = scene
     start = perturbed_start(s.truth, 3, 0.05)
     g = scene_group(s, tgt, values=start)
-    sg = SparseGroup(g, g.window)
+    sg = SparseGroup(g, g.target[g.window].window)
     rep = g.parameters.vector_representation().clone()
     Y = torch.tensor(data, dtype=torch.float32, device="cuda").reshape(-1)
     W = torch.ones_like(Y)

@@ -71,7 +71,7 @@ def test_scene_group_takes_the_sparse_route():
     s, tgt = small_scene()
     g = scene_group(s, tgt)
     assert g.parameters.size == 6 * 7 + 1
-    assert SparseGroup.unsupported(g, g.window) is None
+    assert SparseGroup.unsupported(g, g.target[g.window].window) is None
     # the last source's parameters sit just before the sky's
     assert list(g.parameters.vector_names())[-8:-1] == ["center:0", "center:1", "q", "PA", "n", "Re", "Ie"]
 
@@ -80,16 +80,39 @@ def test_unsupported_groups_say_why():
     s, tgt = small_scene()
     g = scene_group(s, tgt)
     g.models["src2"]["n"].locked = True
-    assert "free" in SparseGroup.unsupported(g, g.window)
+    assert "free" in SparseGroup.unsupported(g, g.target[g.window].window)
     g.models["src2"]["n"].locked = False
     g.models["src3"].psf_mode = "none"
-    assert "psf_mode" in SparseGroup.unsupported(g, g.window)
+    assert "psf_mode" in SparseGroup.unsupported(g, g.target[g.window].window)
     g.models["src3"].psf_mode = "full"
     g.add_model(ap.models.AstroPhot_Model(name="m", model_type="moffat galaxy model", target=tgt,
                                           parameters={"center": [50, 50], "q": 0.5, "PA": 1.0, "n": 2.0,
                                                       "Rd": 3.0, "I0": 0.0}))
-    assert "only Sersic" in SparseGroup.unsupported(g, g.window)
+    assert "only Sersic" in SparseGroup.unsupported(g, g.target[g.window].window)
     single = ap.models.AstroPhot_Model(name="x", model_type="sersic galaxy model", target=tgt,
                                        parameters={"center": [50, 50], "q": 0.5, "PA": 1.0, "n": 2.0,
                                                    "Re": 3.0, "Ie": 0.0})
     assert SparseGroup.unsupported(single, single.window) == "not a group model"
+
+
+def test_group_window_union_follows_reference_rounding():
+    """window_object.py:483-495 takes the union's end as max(pixel_shape,
+    other's far corner in pixel coordinates) while corners sit at n - 0.5, and
+    rounds the shape half to even: an odd-width window united with one that
+    ends inside it grows by a pixel.  The block-sparse fit runs on the
+    target's part of such a group window."""
+    tgt = ap.image.Target_Image(data=np.zeros((40, 80)), pixelscale=1.0)
+    a = tgt.window.copy().crop_to_pixel([[0, 67], [0, 40]])
+    b = tgt.window.copy().crop_to_pixel([[10, 30], [5, 20]])
+    u = a | b
+    # origin stays at -0.5, end = max(67, 29.5) = 67 -> shape 67.5 -> 68 (half to even)
+    assert (u.nx, u.ny) == (68, 40)
+    a2 = tgt.window.copy().crop_to_pixel([[0, 66], [0, 40]])
+    assert ((a2 | b).nx, (a2 | b).ny) == (66, 40)
+    s = group_scene(30, 20261205, side=300, psf_side=11, min_half=16, re_range=(1.5, 5.0))
+    t = ap.image.Target_Image(data=np.zeros((300, 300)), pixelscale=1.0, variance=np.ones((300, 300)),
+                              psf=ap.image.PSF_Image(data=s.psf, pixelscale=1.0))
+    g = scene_group(s, t)
+    grid = g.target[g.window].window
+    assert (grid.nx, grid.ny) == (300, 300)
+    assert SparseGroup.unsupported(g, grid) is None

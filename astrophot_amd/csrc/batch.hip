@@ -273,6 +273,39 @@ __global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, in
 constexpr double kBandRel = 1e-4;
 
 // Gauss-Legendre table of order n and the sub-gridding g
+// Every galaxy of a batch is a Sersic profile (sersic_model.py:41-91,
+// parametric_profiles.py:7-24).  Its float32 samples use the raw v_log_f32 /
+// v_exp_f32 / v_rsq_f32 units in base 2, and the value needs no square root:
+//   log2(R/Re) = log2(R^2)/2 + log2(1/Re),   (R/Re)^(1/n) = 2^(log2(R/Re)/n),
+//   I = scale amp 2^(-b log2(e) ((R/Re)^(1/n) - 1)),
+// with R^2 = X'^2 + Y'^2 + softening^2 after the rotation and the 1/q stretch
+// (the same quantities as eval_profile, which the float64 decisions keep).
+template <bool GRAD>
+__device__ __forceinline__ float sersic_f(const Prof<float>& p, float X, float Y, float scale, float* g) {
+  const float Xr = p.c * X - p.s * Y;
+  const float Yr = (p.s * X + p.c * Y) * p.invq;
+  const float R2 = Xr * Xr + Yr * Yr + p.soft2;
+  const float l2 = 0.5f * __builtin_amdgcn_logf(R2) + p.l2s;
+  const float pw = __builtin_amdgcn_exp2f(l2 * p.inva);
+  const float I = p.amp * scale * __builtin_amdgcn_exp2f(p.nbl2e * (pw - 1.0f));
+  if (GRAD) {
+    const float invR = __builtin_amdgcn_rsqf(R2);
+    const float lx = l2 * 0.693147180559945309f;  // ln(R/Re)
+    const float t = I * p.b * pw * p.inva;
+    const float dIdR = -t * invR;
+    const float dRdX = (p.c * Xr + p.s * Yr * p.invq) * invR;
+    const float dRdY = (-p.s * Xr + p.c * Yr * p.invq) * invR;
+    g[APX_P_CX] = -dIdR * dRdX;  // X = coordinate - centre
+    g[APX_P_CY] = -dIdR * dRdY;
+    g[APX_P_Q] = -dIdR * Yr * Yr * p.invq * invR;
+    g[APX_P_PA] = dIdR * Xr * Yr * p.qm * invR;
+    g[APX_P_A] = I * (-p.db * (pw - 1.0f) + p.b * pw * lx * p.inva * p.inva);
+    g[APX_P_B] = t * p.inv_s;
+    g[APX_P_C] = I * (float)kLn10;
+  }
+  return I;
+}
+
 struct QuadTab {
   int n, nn, g, gg, max_depth;
   float node[APX_MAX_QUAD_LEVEL];
@@ -283,7 +316,8 @@ struct QuadTab {
 
 // n x n Gauss-Legendre sample of the square of half-width h (in pixel-matrix
 // units) centred on (X, Y) (operations.py:123-147); `centre` is the middle node
-template <bool GRAD>
+// QN: the quadrature level when known at compile time (0: t.n)
+template <bool GRAD, int QN = 0>
 __device__ __forceinline__ float quad_f(const Prof<float>& p, const QuadTab& t, float X, float Y,
                                         float h, float scale, float* g, float& centre) {
   float res = 0.0f, gl[APX_NSLOT];
@@ -292,12 +326,13 @@ __device__ __forceinline__ float quad_f(const Prof<float>& p, const QuadTab& t, 
     for (int q = 0; q < APX_NSLOT; ++q) g[q] = 0.0f;
   }
   const float h00 = h * p.P00, h01 = h * p.P01, h10 = h * p.P10, h11 = h * p.P11;
-  for (int k = 0; k < t.nn; ++k) {
-    const float u = t.node[k % t.n], v = t.node[k / t.n], w = t.w2[k];
-    const float val =
-        eval_profile<float, GRAD>(p, X + h00 * u + h01 * v, Y + h10 * u + h11 * v, scale, gl);
+  const int n = QN ? QN : t.n, nn = n * n;
+#pragma unroll
+  for (int k = 0; k < nn; ++k) {
+    const float u = t.node[k % n], v = t.node[k / n], w = t.w2[k];
+    const float val = sersic_f<GRAD>(p, X + h00 * u + h01 * v, Y + h10 * u + h11 * v, scale, gl);
     res = fmaf(w, val, res);
-    if (k == t.nn / 2) centre = val;
+    if (k == nn / 2) centre = val;
     if (GRAD) {
 #pragma unroll
       for (int q = 0; q < APX_NSLOT; ++q) g[q] = fmaf(w, gl[q], g[q]);
@@ -340,7 +375,7 @@ __global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s) {
     if (pix >= E.nxw * E.nyw) continue;
     float X, Y, gr[APX_NSLOT];
     pixel_xy(E.pf, E.ci, E.cj, pix % E.nxw, pix / E.nxw, X, Y);
-    const float val = eval_profile<float, GRAD>(E.pf, X, Y, 1.0f, gr);
+    const float val = sersic_f<GRAD>(E.pf, X, Y, 1.0f, gr);
     s.mid[G.work_off + pix] = val;
     if (GRAD) {
       // planes 1..5: q, PA, n, Re, Ie in representation units
@@ -360,7 +395,7 @@ enum : int { PX_DONE = 0, PX_DEEPER = 1, PX_UNDECIDED = 2 };
 
 // curvature selection (_model_methods.py:87-97, 155-166) of working pixel
 // `pix` and, if selected, its level-1 quadrature (operations.py:210-228)
-template <bool GRAD, bool F64>
+template <bool GRAD, bool F64, int QN>
 __device__ __forceinline__ int select_pixel(const apx_batch& b, const Scratch& s,
                                             const apx_batch_galaxy& G, const GalEval& E,
                                             const QuadTab& t, int pix) {
@@ -393,7 +428,7 @@ __device__ __forceinline__ int select_pixel(const apx_batch& b, const Scratch& s
   }
   float X0, Y0, gs[APX_NSLOT], centre = 0.0f;
   pixel_xy(E.pf, E.ci, E.cj, i, j, X0, Y0);
-  const float r1 = quad_f<GRAD>(E.pf, t, X0, Y0, 0.5f, 1.0f, gs, centre);
+  const float r1 = quad_f<GRAD, QN>(E.pf, t, X0, Y0, 0.5f, 1.0f, gs, centre);
   if (t.max_depth > 1) {
     const int dec = band_test(r1, centre, E.thr);
     bool deeper = dec == 1;
@@ -442,7 +477,7 @@ __device__ __forceinline__ int locate(const int32_t* off, int ngal, int k) {
 }
 
 // float32 selection of every listed chunk, one lane per pixel
-template <bool GRAD>
+template <bool GRAD, int QN>
 __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, QuadTab t) {
   const int lane = threadIdx.x & 63;
   for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
@@ -450,21 +485,21 @@ __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, Qua
     const apx_batch_galaxy G = b.gal[e.x];
     const GalEval E = s.ev[e.x];
     const int pix = e.y * kChunk + threadIdx.x;
-    const int res = pix < E.nxw * E.nyw ? select_pixel<GRAD, false>(b, s, G, E, t, pix) : PX_DONE;
+    const int res = pix < E.nxw * E.nyw ? select_pixel<GRAD, false, QN>(b, s, G, E, t, pix) : PX_DONE;
     wave_append(res == PX_DEEPER, &s.gal_cnt[e.x], s.deep + G.work_off, pix, lane);
     wave_append(res == PX_UNDECIDED, &s.gal_rcnt[e.x], s.redo + G.work_off, pix, lane);
   }
 }
 
 // the pixels select_kernel left undecided, with float64 decisions
-template <bool GRAD>
+template <bool GRAD, int QN>
 __global__ __launch_bounds__(256) void select_redo_kernel(apx_batch b, Scratch s, QuadTab t) {
   const int total = s.cnt[4];
   for (int k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
     const int g = locate(s.gal_roff, b.ngal, k);
     const apx_batch_galaxy G = b.gal[g];
     const int pix = s.redo[G.work_off + (k - s.gal_roff[g])];
-    if (select_pixel<GRAD, true>(b, s, G, s.ev[g], t, pix) == PX_DEEPER)
+    if (select_pixel<GRAD, true, QN>(b, s, G, s.ev[g], t, pix) == PX_DEEPER)
       s.deep[G.work_off + atomicAdd(&s.gal_cnt[g], 1)] = pix;
   }
 }
@@ -512,17 +547,41 @@ __global__ __launch_bounds__(1024) void scan_kernel(int n, int32_t* cnt, int32_t
 // level-2 test flags is then integrated by the whole group.  F64 false: the
 // deep list, and a pixel with an undecided sub-pixel test goes to the redo
 // list unwritten; F64 true: the redo list, float64 decisions.
-template <bool GRAD, int GS, bool F64>
+// QN, QG: quadrature level and gridding when known at compile time (0: from
+// t).  With both known, each lane's level-3 sample offsets (plane units) and
+// weights are computed once per launch.
+template <bool GRAD, int GS, bool F64, int QN, int QG>
 __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, QuadTab t) {
   constexpr int NG = 64 / GS;
+  constexpr bool FIXED = QN > 0 && QG > 0;
+  constexpr int ITS = FIXED ? (QN * QN * QG * QG + GS - 1) / GS : 1;
   const int lane = threadIdx.x & 63, gl = lane % GS, grp = lane / GS;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
   const int total = F64 ? s.cnt[5] : s.cnt[3];
   const int32_t* off = F64 ? s.gal_roff : s.gal_off;
   const int32_t* list = F64 ? s.redo : s.deep;
-  const int g_ = t.g, gg = t.gg, nq = gg * t.nn;
+  const int qn = QN ? QN : t.n, qnn = qn * qn;
+  const int g_ = QG ? QG : t.g, gg = g_ * g_, nq = gg * qnn;
   const float ig = 1.0f / (float)g_;
   const float sub0 = -(float)(g_ - 1) / (2.0f * (float)g_), subd = ig;
+  const float h3 = 0.5f * ig * ig;
+  float l3x[ITS], l3y[ITS], l3w[ITS];
+  if (FIXED) {
+    const float P00 = (float)b.P[0], P01 = (float)b.P[1], P10 = (float)b.P[2], P11 = (float)b.P[3];
+#pragma unroll
+    for (int it = 0; it < ITS; ++it) {
+      const int e = gl + GS * it;
+      l3x[it] = l3y[it] = l3w[it] = 0.0f;
+      if (e < nq) {
+        const int tt = e / qnn, kk = e % qnn;
+        const float a = (sub0 + subd * (float)(tt % g_)) * ig, c = (sub0 + subd * (float)(tt / g_)) * ig;
+        const float u = a + h3 * t.node[kk % qn], v = c + h3 * t.node[kk / qn];
+        l3x[it] = P00 * u + P01 * v;
+        l3y[it] = P10 * u + P11 * v;
+        l3w[it] = t.w2[kk];
+      }
+    }
+  }
   const uint64_t gmask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << (grp * GS));
   for (int k0 = wave * NG; k0 < total; k0 += nwaves * NG) {
     const int k = k0 + grp;
@@ -549,7 +608,7 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
       x2 = X0 + pf.P00 * a + pf.P01 * c;
       y2 = Y0 + pf.P10 * a + pf.P11 * c;
       float c2 = 0.0f;
-      r2 = quad_f<GRAD>(pf, t, x2, y2, 0.5f * ig, ig * ig, gs, c2);
+      r2 = quad_f<GRAD, QN>(pf, t, x2, y2, 0.5f * ig, ig * ig, gs, c2);
       if (t.max_depth > 2) {
         const double thr2 = E->thr * (double)gg;
         dec = band_test(r2, c2, thr2);
@@ -576,7 +635,7 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
     }
     // level 3: the group integrates each flagged sub-pixel in turn
     uint64_t todo = (__ballot(deeper) & gmask) >> (GS == 64 ? 0 : grp * GS);
-    const float sc3 = ig * ig * ig * ig, h3 = 0.5f * ig * ig;
+    const float sc3 = ig * ig * ig * ig;
     while (__ballot(todo != 0ull)) {
       int sp = -1;
       if (todo) {
@@ -586,18 +645,34 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
       const int src = grp * GS + (sp < 0 ? 0 : sp);
       const float xs = __shfl(x2, src, 64), ys = __shfl(y2, src, 64);
       if (sp >= 0) {
-        for (int e = gl; e < nq; e += GS) {
-          const int tt = e / t.nn, kk = e % t.nn;
-          const float a = (sub0 + subd * (float)(tt % g_)) * ig, c = (sub0 + subd * (float)(tt / g_)) * ig;
-          const float u = a + h3 * t.node[kk % t.n], v = c + h3 * t.node[kk / t.n];
-          float gv[APX_NSLOT];
-          const float val = eval_profile<float, GRAD>(pf, xs + pf.P00 * u + pf.P01 * v,
-                                                      ys + pf.P10 * u + pf.P11 * v, sc3, gv);
-          const double w = (double)t.w2[kk];
-          acc[0] += w * (double)val;
-          if (GRAD) {
+        if (FIXED) {
 #pragma unroll
-            for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] += w * (double)gv[q];
+          for (int it = 0; it < ITS; ++it) {
+            if (gl + GS * it < nq) {
+              float gv[APX_NSLOT];
+              const float val = sersic_f<GRAD>(pf, xs + l3x[it], ys + l3y[it], sc3, gv);
+              const double w = (double)l3w[it];
+              acc[0] += w * (double)val;
+              if (GRAD) {
+#pragma unroll
+                for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] += w * (double)gv[q];
+              }
+            }
+          }
+        } else {
+          for (int e = gl; e < nq; e += GS) {
+            const int tt = e / qnn, kk = e % qnn;
+            const float a = (sub0 + subd * (float)(tt % g_)) * ig, c = (sub0 + subd * (float)(tt / g_)) * ig;
+            const float u = a + h3 * t.node[kk % qn], v = c + h3 * t.node[kk / qn];
+            float gv[APX_NSLOT];
+            const float val = sersic_f<GRAD>(pf, xs + pf.P00 * u + pf.P01 * v, ys + pf.P10 * u + pf.P11 * v,
+                                              sc3, gv);
+            const double w = (double)t.w2[kk];
+            acc[0] += w * (double)val;
+            if (GRAD) {
+#pragma unroll
+              for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] += w * (double)gv[q];
+            }
           }
         }
       }
@@ -780,22 +855,54 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
 // block of the 64 x kTR output tile with v_mfma_f32_16x16x4f32 as
 //   out[r][x] = sum_kx sum_j A_kx[x][j] B_kx[j][r],
 //   A_kx[x][j] = in[j][x + kx],  B_kx[j][r] = Kf[j - r][kx] (zero outside the kernel),
-// j over the staged input rows in k-steps of 4, grouped by 4.  A comes from the staged
-// input tile, B from a per-kernel tap table tab[kx][dd] = Kf[dd - 15][kx], both
-// in LDS.  Lane layout: A[l % 16][l / 16], B[l / 16][l % 16], and
+// j over the staged input rows in k-steps of 4.  A comes from the staged input
+// tile, B from a per-kernel tap table tab[kx][dd] = Kf[dd - 15][kx], both in
+// LDS.  Lane layout: A[l % 16][l / 16], B[l / 16][l % 16], and
 // D[4 (l / 16) + i][l % 16] in acc[i] (x = 4 (l / 16) + i, r = l % 16).
-// k-steps of 4 input rows come in groups of 4 (16 rows); the staged rows and
-// the tap table are padded with zeros to whole groups
-__host__ __device__ inline int mf_ng(int kh) { return (kTR + kh - 1 + 15) / 16; }  // groups per kx
-__host__ __device__ inline int mf_rows(int kh) { return 16 * mf_ng(kh); }
+// A kernel column needs kTR + kh - 1 input rows, ceil of that / 4 k-steps
+// (a compile-time count per tile, dispatched on the PSF size); the staged rows
+// and the tap table are padded with zeros to whole k-steps.
+__host__ __device__ inline int mf_nks(int kh) { return (kTR + kh - 1 + 3) / 4; }  // k-steps per kx
+__host__ __device__ inline int mf_rows(int kh) { return 4 * mf_nks(kh); }
 __host__ __device__ inline int mf_lw(int kw) { return (64 + kw - 1) | 1; }  // odd pitch
 __host__ __device__ inline int mf_dd(int kh) { return mf_rows(kh) + 15; }
 __host__ inline size_t mfma_lds_bytes(int kmax, int nk) {
   return sizeof(float) * ((size_t)mf_rows(kmax) * mf_lw(kmax) + (size_t)nk * kmax * mf_dd(kmax));
 }
 
+// All kernel columns of one output block: column kx needs NKS k-steps; the
+// operands of the next column are loaded while the MFMAs of the current one
+// issue (two register sets of NKS A and B values).
+template <int NKS>
+__device__ __forceinline__ v4f mfma_columns(const float* ta, const float* tb, int kw, int LW, int DD) {
+  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  float a0[NKS], b0[NKS], a1[NKS], b1[NKS];
+  auto load = [&](int kx, float (&a)[NKS], float (&bb)[NKS]) {
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) {
+      a[t] = ta[kx + 4 * t * LW];
+      bb[t] = tb[kx * DD + 4 * t];
+    }
+  };
+  load(0, a0, b0);
+  int kx = 0;
+  for (; kx + 1 < kw; kx += 2) {
+    load(kx + 1, a1, b1);
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
+    if (kx + 2 < kw) load(kx + 2, a0, b0);
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[t], b1[t], acc, 0, 0, 0);
+  }
+  if (kx < kw) {
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(256) void conv_mfma_kernel(apx_batch b, Scratch s, float* out) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void conv_mfma_kernel(apx_batch b, Scratch s, float* out) {
   extern __shared__ float smem[];
   __shared__ double redd[4][8];  // kConvStaticLds
   constexpr bool JAC = MODE == EPI_NORMAL;
@@ -813,7 +920,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(apx_batch b, Scratch s, 
     const int ntx = (G.nx + 63) / 64;
     const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kTR;
     const int kh = G.ph + 2, kw = G.pw + 2;
-    const int ng = mf_ng(kh), LW = mf_lw(kw), DD = mf_dd(kh), kts = fast_kts(kh);
+    const int nks = mf_nks(kh), LW = mf_lw(kw), DD = mf_dd(kh), kts = fast_kts(kh);
     const int rows = mf_rows(kh), cols = 64 + kw - 1;
     const int64_t tile = G.tile_off + e.y;
     float* tin = smem;             // rows x LW
@@ -849,37 +956,19 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(apx_batch b, Scratch s, 
       __syncthreads();
       const int nk = (JAC && c == 0) ? 3 : 1;
       for (int kk = 0; kk < nk; ++kk) {
-        v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
         const float* ta = tin + lk * LW + 16 * w + lm;
         const float* tb = tab + kk * kw * DD + lk - lm + 15;
-        // group q = (kx, jg): k-steps jj = 4 jg .. 4 jg + 3 of column kx.  Two
-        // operand sets alternate: the loads of the next group are in flight
-        // while the MFMAs of the current one issue.
-        const int nq = kw * ng;
-        auto load = [&](int q, float (&a)[4], float (&bb)[4]) {
-          const int kx = q / ng, jg = q - kx * ng;
-          const float* pa = ta + kx + 16 * jg * LW;
-          const float* pb = tb + kx * DD + 16 * jg;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            a[t] = pa[4 * t * LW];
-            bb[t] = pb[4 * t];
-          }
-        };
-        float a0[4], b0[4], a1[4], b1[4];
-        load(0, a0, b0);
-        int q = 0;
-        for (; q + 1 < nq; q += 2) {
-          load(q + 1, a1, b1);
-#pragma unroll
-          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
-          if (q + 2 < nq) load(q + 2, a0, b0);
-#pragma unroll
-          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[t], b1[t], acc, 0, 0, 0);
-        }
-        if (q < nq) {
-#pragma unroll
-          for (int t = 0; t < 4; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
+        v4f acc;
+        switch (nks) {  // compile-time k-step count per kernel column (kh <= kMfmaMax)
+          case 5: acc = mfma_columns<5>(ta, tb, kw, LW, DD); break;
+          case 6: acc = mfma_columns<6>(ta, tb, kw, LW, DD); break;
+          case 7: acc = mfma_columns<7>(ta, tb, kw, LW, DD); break;
+          case 8: acc = mfma_columns<8>(ta, tb, kw, LW, DD); break;
+          case 9: acc = mfma_columns<9>(ta, tb, kw, LW, DD); break;
+          case 10: acc = mfma_columns<10>(ta, tb, kw, LW, DD); break;
+          case 11: acc = mfma_columns<11>(ta, tb, kw, LW, DD); break;
+          case 12: acc = mfma_columns<12>(ta, tb, kw, LW, DD); break;
+          default: acc = mfma_columns<13>(ta, tb, kw, LW, DD); break;
         }
         const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
 #pragma unroll
@@ -1327,16 +1416,41 @@ QuadTab quad_table(const apx_batch& b) {
   return t;
 }
 
-template <int GS>
+template <int GS, int QN, int QG>
 void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac,
                       hipStream_t stream) {
   const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
-  if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, false>), ig, dim3(256), 0, stream, b, s, t);
-  else hipLaunchKernelGGL((integrate_kernel<false, GS, false>), ig, dim3(256), 0, stream, b, s, t);
+  if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
+  else hipLaunchKernelGGL((integrate_kernel<false, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
                      s.cnt + 5, 0);
-  if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true>), ig, dim3(256), 0, stream, b, s, t);
-  else hipLaunchKernelGGL((integrate_kernel<false, GS, true>), ig, dim3(256), 0, stream, b, s, t);
+  if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
+  else hipLaunchKernelGGL((integrate_kernel<false, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
+}
+
+// the render launches after prepare, for quadrature level QN and gridding QG
+// fixed at compile time (the defaults, 3 and 5) or taken from t (0, 0)
+template <int QN, int QG>
+void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac, int pc,
+                     hipStream_t stream) {
+  if (jac) {
+    hipLaunchKernelGGL(init_kernel<true>, dim3(pc), dim3(kChunk), 0, stream, b, s);
+    hipLaunchKernelGGL((select_kernel<true, QN>), dim3(pc), dim3(256), 0, stream, b, s, t);
+  } else {
+    hipLaunchKernelGGL(init_kernel<false>, dim3(pc), dim3(kChunk), 0, stream, b, s);
+    hipLaunchKernelGGL((select_kernel<false, QN>), dim3(pc), dim3(256), 0, stream, b, s, t);
+  }
+  // float64 redo of undecided selections, then the level-2/3 quadrature of
+  // the deep list, then its undecided pixels in float64
+  const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
+                     s.cnt + 4, 1);
+  if (jac) hipLaunchKernelGGL((select_redo_kernel<true, QN>), ig, dim3(256), 0, stream, b, s, t);
+  else hipLaunchKernelGGL((select_redo_kernel<false, QN>), ig, dim3(256), 0, stream, b, s, t);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_cnt, s.gal_off,
+                     s.cnt + 3, 0);
+  if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32, QN, QG>(b, s, t, jac, stream);
+  else launch_integrate<64, QN, QG>(b, s, t, jac, stream);
 }
 
 // work lists + render planes of the fits active in `stage`
@@ -1350,24 +1464,8 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
                      rep_override);
   const int pc = (int)std::min<int64_t>(b.nchunk, b.chunk_blocks > 0 ? b.chunk_blocks : 4 * kPersist);
   const QuadTab t = quad_table(b);
-  if (jac) {
-    hipLaunchKernelGGL(init_kernel<true>, dim3(pc), dim3(kChunk), 0, stream, b, s);
-    hipLaunchKernelGGL(select_kernel<true>, dim3(pc), dim3(256), 0, stream, b, s, t);
-  } else {
-    hipLaunchKernelGGL(init_kernel<false>, dim3(pc), dim3(kChunk), 0, stream, b, s);
-    hipLaunchKernelGGL(select_kernel<false>, dim3(pc), dim3(256), 0, stream, b, s, t);
-  }
-  // float64 redo of undecided selections, then the level-2/3 quadrature of
-  // the deep list, then its undecided pixels in float64
-  const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
-                     s.cnt + 4, 1);
-  if (jac) hipLaunchKernelGGL(select_redo_kernel<true>, ig, dim3(256), 0, stream, b, s, t);
-  else hipLaunchKernelGGL(select_redo_kernel<false>, ig, dim3(256), 0, stream, b, s, t);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_cnt, s.gal_off,
-                     s.cnt + 3, 0);
-  if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32>(b, s, t, jac, stream);
-  else launch_integrate<64>(b, s, t, jac, stream);
+  if (t.n == 3 && t.g == 5) render_launches<3, 5>(b, s, t, jac, pc, stream);
+  else render_launches<0, 0>(b, s, t, jac, pc, stream);
 }
 
 // convolution of the listed tiles (epilogue `mode`), then for the LM stages

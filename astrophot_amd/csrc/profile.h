@@ -47,6 +47,7 @@ struct Prof {
   T soft2;
   T amp;                          // level-1 pixel_area * 10**I
   T a, inva, b, db, binva, inv_s; // profile-specific
+  T l2s, nbl2e;                   // Sersic: log2(1/Re), -b log2(e)
 };
 
 template <typename T>
@@ -68,7 +69,7 @@ __device__ inline Prof<T> make_prof(int profile, const double* P, double fx, dou
   p.invq = (T)(1.0 / q);
   p.qm = (T)(q - 1.0 / q);
   p.soft2 = (T)(softening * softening);
-  p.a = p.inva = p.b = p.db = p.binva = p.inv_s = (T)0;
+  p.a = p.inva = p.b = p.db = p.binva = p.inv_s = p.l2s = p.nbl2e = (T)0;
   switch (profile) {
     case APX_SERSIC: {  // prm: n, Re, Ie
       const double n = prm[0];
@@ -78,6 +79,8 @@ __device__ inline Prof<T> make_prof(int profile, const double* P, double fx, dou
       p.b = (T)sersic_b(n);
       p.db = (T)sersic_db(n);
       p.inv_s = (T)(1.0 / prm[1]);  // 1/Re
+      p.l2s = (T)(-log2(prm[1]));
+      p.nbl2e = (T)(-sersic_b(n) * 1.44269504088896340736);
       break;
     }
     case APX_EXPONENTIAL: {  // prm: Re, Ie ; b(1) from the Sersic polynomial

@@ -62,6 +62,7 @@ struct GalEval {
   Prof<double> pd;
   double thr;
   double dvdr[NP];
+  double sx, sy;  // sub-pixel shift of the PSF (model_object.py:320-325)
   int32_t nxw, nyw, ci, cj, bx, by;
 };
 
@@ -80,7 +81,10 @@ struct Scratch {
   float* mid;         // nwork: midpoint samples
   double* part;       // ntile * kPart: per-tile partial sums of the reductions
   GalEval* ev;        // ngal
+  int32_t* bsum;      // ceil(ngal / kScanBlock): block totals of the count scans
 };
+
+constexpr int kScanBlock = 1024;  // counts per workgroup of the two-pass scan
 
 constexpr int kPart = 64;  // doubles per tile partial (49 H + 7 g + 1 chi2 fit the largest)
 
@@ -108,6 +112,7 @@ __host__ inline int64_t scratch_layout(const apx_batch& b, Scratch* s) {
   t.mid = reinterpret_cast<float*>(take(4 * b.nwork));
   t.part = reinterpret_cast<double*>(take(8 * kPart * b.ntile));
   t.ev = reinterpret_cast<GalEval*>(take((int64_t)sizeof(GalEval) * b.ngal));
+  t.bsum = reinterpret_cast<int32_t*>(take(4 * (((int64_t)b.ngal + kScanBlock - 1) / kScanBlock)));
   if (s) *s = t;
   return off;
 }
@@ -178,44 +183,54 @@ __device__ inline void rep_to_values(const double* r, double* v, double* dvdr) {
 }
 
 // per fit: render constants + shifted PSF taps in the conv layout
-__global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, int stage,
-                                                      const double* rep_override) {
+// per fit (one thread each): render constants at the fit's parameters
+__global__ __launch_bounds__(256) void eval_kernel(apx_batch b, Scratch s, const double* rep_override) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= s.cnt[0]) return;
+  const int g = s.gal_list[k];
+  const apx_batch_galaxy G = b.gal[g];
+  const double* rep = rep_override ? rep_override + (int64_t)g * NP : st_of(b, g)->ev;
+  double v[NP], dvdr[NP];
+  rep_to_values(rep, v, dvdr);
+  const int bx = (G.pw + 1) / 2, by = (G.ph + 1) / 2;  // psf_image.py:71-93
+  const int nxw = G.nx + 2 * bx, nyw = G.ny + 2 * by;
+  const double det = b.P[0] * b.P[3] - b.P[1] * b.P[2];
+  const double Pi[4] = {b.P[3] / det, -b.P[1] / det, -b.P[2] / det, b.P[0] / det};
+  const double dx = v[0] - G.origin[0], dy = v[1] - G.origin[1];
+  const double pcx = Pi[0] * dx + Pi[1] * dy + (-0.5 + bx);
+  const double pcy = Pi[2] * dx + Pi[3] * dy + (-0.5 + by);
+  const double ci = rint(pcx), cj = rint(pcy);  // ties to even
+  const double prm[3] = {v[4], v[5], v[6]};
+  const double area = fabs(det);
+  GalEval e;
+  e.pf = make_prof<float>(APX_SERSIC, b.P, 0.0, 0.0, area, b.softening, v[2], v[3], prm);
+  e.pd = make_prof<double>(APX_SERSIC, b.P, 0.0, 0.0, area, b.softening, v[2], v[3], prm);
+  const double bn = sersic_b(v[4]);
+  const double flux = 2.0 * kPi * pow(10.0, v[6]) * v[5] * v[5] * v[2] * v[4] * exp(bn) *
+                      pow(bn, -2.0 * v[4]) * exp(lgamma(2.0 * v[4]));  // functions.py:168-190
+  e.thr = b.tolerance * flux / (G.ref_npix > 0 ? (double)G.ref_npix : (double)nxw * nyw);
+  for (int q = 0; q < NP; ++q) e.dvdr[q] = dvdr[q];
+  e.sx = pcx - ci;
+  e.sy = pcy - cj;
+  e.nxw = nxw;
+  e.nyw = nyw;
+  e.ci = (int)ci;
+  e.cj = (int)cj;
+  e.bx = bx;
+  e.by = by;
+  s.ev[g] = e;
+}
+
+// per fit (one workgroup each): shifted PSF taps in the conv layout
+__global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, int stage) {
   __shared__ double red[4];
   const int nk = (stage == ST_JAC || stage == ST_ALL_JAC) ? 3 : 1;
+  const double det = b.P[0] * b.P[3] - b.P[1] * b.P[2];
+  const double Pi[4] = {b.P[3] / det, -b.P[1] / det, -b.P[2] / det, b.P[0] / det};
   for (int k = blockIdx.x; k < s.cnt[0]; k += gridDim.x) {
     const int g = s.gal_list[k];
     const apx_batch_galaxy G = b.gal[g];
-    const double* rep = rep_override ? rep_override + (int64_t)g * NP : st_of(b, g)->ev;
-    double v[NP], dvdr[NP];
-    rep_to_values(rep, v, dvdr);
-    const int bx = (G.pw + 1) / 2, by = (G.ph + 1) / 2;  // psf_image.py:71-93
-    const int nxw = G.nx + 2 * bx, nyw = G.ny + 2 * by;
-    const double det = b.P[0] * b.P[3] - b.P[1] * b.P[2];
-    const double Pi[4] = {b.P[3] / det, -b.P[1] / det, -b.P[2] / det, b.P[0] / det};
-    const double dx = v[0] - G.origin[0], dy = v[1] - G.origin[1];
-    const double pcx = Pi[0] * dx + Pi[1] * dy + (-0.5 + bx);
-    const double pcy = Pi[2] * dx + Pi[3] * dy + (-0.5 + by);
-    const double ci = rint(pcx), cj = rint(pcy);  // ties to even
-    const double sx = pcx - ci, sy = pcy - cj;
-    const double prm[3] = {v[4], v[5], v[6]};
-    const double area = fabs(det);
-    if (threadIdx.x == 0) {
-      GalEval e;
-      e.pf = make_prof<float>(APX_SERSIC, b.P, 0.0, 0.0, area, b.softening, v[2], v[3], prm);
-      e.pd = make_prof<double>(APX_SERSIC, b.P, 0.0, 0.0, area, b.softening, v[2], v[3], prm);
-      const double bn = sersic_b(v[4]);
-      const double flux = 2.0 * kPi * pow(10.0, v[6]) * v[5] * v[5] * v[2] * v[4] * exp(bn) *
-                          pow(bn, -2.0 * v[4]) * exp(lgamma(2.0 * v[4]));  // functions.py:168-190
-      e.thr = b.tolerance * flux / (G.ref_npix > 0 ? (double)G.ref_npix : (double)nxw * nyw);
-      for (int q = 0; q < NP; ++q) e.dvdr[q] = dvdr[q];
-      e.nxw = nxw;
-      e.nyw = nyw;
-      e.ci = (int)ci;
-      e.cj = (int)cj;
-      e.bx = bx;
-      e.by = by;
-      s.ev[g] = e;
-    }
+    const double sx = s.ev[g].sx, sy = s.ev[g].sy;
     // shifted, normalised PSF (and its centre derivatives), flipped, in the
     // KT[k][b][u] = Kf[u - 7][b] layout of conv_common.h
     const int H2 = G.ph + 2, W2 = G.pw + 2, n = H2 * W2, kts = fast_kts(H2);
@@ -504,19 +519,55 @@ __global__ __launch_bounds__(256) void select_redo_kernel(apx_batch b, Scratch s
   }
 }
 
-// exclusive prefix of n per-fit counts (one workgroup of 1024): off[0..n],
-// *total = off[n]; the counts are cleared after reading when `clear`.  Each
-// thread owns a contiguous run of ceil(n / 1024) counts: a serial prefix of
-// its run, one block-wide scan of the run totals, then the run's offsets.
-__global__ __launch_bounds__(1024) void scan_kernel(int n, int32_t* cnt, int32_t* off,
-                                                    int32_t* total, int clear) {
-  __shared__ int32_t wsums[16];
+// exclusive prefix of n per-fit counts: off[0..n], *total = off[n]; the
+// counts are cleared after reading when `clear`.  Two passes over blocks of
+// kScanBlock counts, both coalesced: the block totals, then each block adds
+// the totals of the blocks before it (read in index order) to its own scan.
+__global__ __launch_bounds__(256) void scan_totals_kernel(int n, const int32_t* __restrict__ cnt,
+                                                          int32_t* __restrict__ bsum) {
+  __shared__ int32_t red[4];
+  const int base = blockIdx.x * kScanBlock;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 256; ++k) {
+    const int g = base + k * 256 + (int)threadIdx.x;
+    v += g < n ? cnt[g] : 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void scan_apply_kernel(int n, int32_t* __restrict__ cnt,
+                                                         int32_t* __restrict__ off,
+                                                         int32_t* __restrict__ total,
+                                                         const int32_t* __restrict__ bsum, int clear) {
+  __shared__ __align__(16) int32_t vals[kScanBlock];
+  __shared__ int32_t wsums[4];
+  __shared__ int32_t pre_s;
+  const int nb = gridDim.x, base = blockIdx.x * kScanBlock;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int run = (n + 1023) / 1024;
-  const int g0 = min(n, (int)threadIdx.x * run), g1 = min(n, g0 + run);
-  int sum = 0;
-  for (int g = g0; g < g1; ++g) sum += cnt[g];
-  int x = sum;  // inclusive scan of the run totals
+  // totals of the blocks before this one, summed in index order
+  if (wv == 0) {
+    int p = 0;
+    for (int q = lane; q < (int)blockIdx.x; q += 64) p += bsum[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+    if (lane == 0) pre_s = p;
+  }
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 256; ++k) {
+    const int e = k * 256 + (int)threadIdx.x, g = base + e;
+    vals[e] = g < n ? cnt[g] : 0;
+    if (clear && g < n) cnt[g] = 0;
+  }
+  __syncthreads();
+  // each thread scans 4 consecutive counts; the thread totals scan across the block
+  int4 q4 = *reinterpret_cast<const int4*>(vals + 4 * threadIdx.x);
+  const int tsum = q4.x + q4.y + q4.z + q4.w;
+  int x = tsum;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const int y = __shfl_up(x, o, 64);
@@ -524,22 +575,34 @@ __global__ __launch_bounds__(1024) void scan_kernel(int n, int32_t* cnt, int32_t
   }
   if (lane == 63) wsums[wv] = x;
   __syncthreads();
-  int pre = 0, tot = 0;
-  for (int w = 0; w < 16; ++w) {
-    pre += w < wv ? wsums[w] : 0;
-    tot += wsums[w];
+  int pw = 0;
+  for (int w = 0; w < wv; ++w) pw += wsums[w];
+  int acc = pre_s + pw + x - tsum;
+  int4 o4;
+  o4.x = acc; acc += q4.x;
+  o4.y = acc; acc += q4.y;
+  o4.z = acc; acc += q4.z;
+  o4.w = acc; acc += q4.w;
+  __syncthreads();
+  *reinterpret_cast<int4*>(vals + 4 * threadIdx.x) = o4;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kScanBlock / 256; ++k) {
+    const int e = k * 256 + (int)threadIdx.x, g = base + e;
+    if (g < n) off[g] = vals[e];
   }
-  int acc = pre + x - sum;  // exclusive prefix of this run
-  for (int g = g0; g < g1; ++g) {
-    const int v = cnt[g];
-    off[g] = acc;
-    acc += v;
-    if (clear) cnt[g] = 0;
+  if (blockIdx.x == nb - 1 && threadIdx.x == 255) {
+    off[n] = acc;  // the last thread's running sum is the grand total
+    *total = acc;
   }
-  if (threadIdx.x == 0) {
-    off[n] = tot;
-    *total = tot;
-  }
+}
+
+// the scan of n counts (two launches)
+inline void scan_counts(int n, int32_t* cnt, int32_t* off, int32_t* total, int32_t* bsum, int clear,
+                        hipStream_t stream) {
+  const int nb = (n + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(scan_totals_kernel, dim3(nb), dim3(256), 0, stream, n, cnt, bsum);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(256), 0, stream, n, cnt, off, total, bsum, clear);
 }
 
 // levels 2 and 3 of the quadrature (operations.py:229-247): a group of GS
@@ -1422,8 +1485,7 @@ void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bo
   const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
-                     s.cnt + 5, 0);
+  scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 5, s.bsum, 0, stream);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
 }
@@ -1443,12 +1505,10 @@ void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, boo
   // float64 redo of undecided selections, then the level-2/3 quadrature of
   // the deep list, then its undecided pixels in float64
   const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_rcnt, s.gal_roff,
-                     s.cnt + 4, 1);
+  scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 4, s.bsum, 1, stream);
   if (jac) hipLaunchKernelGGL((select_redo_kernel<true, QN>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((select_redo_kernel<false, QN>), ig, dim3(256), 0, stream, b, s, t);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, stream, b.ngal, s.gal_cnt, s.gal_off,
-                     s.cnt + 3, 0);
+  scan_counts(b.ngal, s.gal_cnt, s.gal_off, s.cnt + 3, s.bsum, 0, stream);
   if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32, QN, QG>(b, s, t, jac, stream);
   else launch_integrate<64, QN, QG>(b, s, t, jac, stream);
 }
@@ -1460,8 +1520,8 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
   const int gb = (b.ngal + 255) / 256;
   hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, stream, s.cnt);
   hipLaunchKernelGGL(build_kernel, dim3(gb), dim3(256), 0, stream, b, s, stage);
-  hipLaunchKernelGGL(prepare_kernel, dim3(std::min(b.ngal, 1024)), dim3(256), 0, stream, b, s, stage,
-                     rep_override);
+  hipLaunchKernelGGL(eval_kernel, dim3((b.ngal + 255) / 256), dim3(256), 0, stream, b, s, rep_override);
+  hipLaunchKernelGGL(prepare_kernel, dim3(std::min(b.ngal, 1024)), dim3(256), 0, stream, b, s, stage);
   const int pc = (int)std::min<int64_t>(b.nchunk, b.chunk_blocks > 0 ? b.chunk_blocks : 4 * kPersist);
   const QuadTab t = quad_table(b);
   if (t.n == 3 && t.g == 5) render_launches<3, 5>(b, s, t, jac, pc, stream);

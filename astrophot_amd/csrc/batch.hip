@@ -614,7 +614,8 @@ inline void scan_counts(int n, int32_t* cnt, int32_t* off, int32_t* total, int32
 // t).  With both known, each lane's level-3 sample offsets (plane units) and
 // weights are computed once per launch.
 template <bool GRAD, int GS, bool F64, int QN, int QG>
-__global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, QuadTab t) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void integrate_kernel(apx_batch b, Scratch s,
+                                                                                                QuadTab t) {
   constexpr int NG = 64 / GS;
   constexpr bool FIXED = QN > 0 && QG > 0;
   constexpr int ITS = FIXED ? (QN * QN * QG * QG + GS - 1) / GS : 1;
@@ -709,18 +710,27 @@ __global__ __launch_bounds__(256) void integrate_kernel(apx_batch b, Scratch s, 
       const float xs = __shfl(x2, src, 64), ys = __shfl(y2, src, 64);
       if (sp >= 0) {
         if (FIXED) {
+          // this lane's at most ITS samples of the sub-pixel, summed in float32,
+          // then into the float64 accumulators
+          float fs[1 + APX_NSLOT];
+#pragma unroll
+          for (int q = 0; q <= APX_NSLOT; ++q) fs[q] = 0.0f;
 #pragma unroll
           for (int it = 0; it < ITS; ++it) {
             if (gl + GS * it < nq) {
               float gv[APX_NSLOT];
               const float val = sersic_f<GRAD>(pf, xs + l3x[it], ys + l3y[it], sc3, gv);
-              const double w = (double)l3w[it];
-              acc[0] += w * (double)val;
+              fs[0] = fmaf(l3w[it], val, fs[0]);
               if (GRAD) {
 #pragma unroll
-                for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] += w * (double)gv[q];
+                for (int q = 0; q < APX_NSLOT; ++q) fs[1 + q] = fmaf(l3w[it], gv[q], fs[1 + q]);
               }
             }
+          }
+          acc[0] += (double)fs[0];
+          if (GRAD) {
+#pragma unroll
+            for (int q = 0; q < APX_NSLOT; ++q) acc[1 + q] += (double)fs[1 + q];
           }
         } else {
           for (int e = gl; e < nq; e += GS) {
@@ -990,10 +1000,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
     float* tab = smem + rows * LW; // [kk][kx][dd]
     const float* kt = b.taps + G.tap_off;
     __syncthreads();  // the previous tile's readers are done with LDS
-    for (int q = threadIdx.x; q < nkt * kw * DD; q += 256) {
-      const int dd = q % DD, kx = (q / DD) % kw, kk = q / (DD * kw);
-      const int a = dd - 15;  // Kf row; KT[kk][kx][u] = Kf[u - 7][kx]
-      tab[q] = (a >= 0 && a < kh) ? kt[((int64_t)kk * kw + kx) * kts + a + 7] : 0.0f;
+    for (int r = w; r < nkt * kw; r += 4) {  // tap-table row (kk, kx) per wave, dd across lanes
+      const float* src = kt + (int64_t)r * kts + 7;  // KT[kk][kx][u] = Kf[u - 7][kx]
+      for (int dd = lane; dd < DD; dd += 64) {
+        const int a = dd - 15;  // Kf row
+        tab[r * DD + dd] = (a >= 0 && a < kh) ? src[a] : 0.0f;
+      }
     }
     // this lane's 4 output pixels: x = 16 w + 4 lk + i, row lm
     const int oy = oy0 + lm;

@@ -937,7 +937,9 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
 // and the tap table are padded with zeros to whole k-steps.
 __host__ __device__ inline int mf_nks(int kh) { return (kTR + kh - 1 + 3) / 4; }  // k-steps per kx
 __host__ __device__ inline int mf_rows(int kh) { return 4 * mf_nks(kh); }
-__host__ __device__ inline int mf_lw(int kw) { return (64 + kw - 1) | 1; }  // odd pitch
+// row pitch of the staged input: 16 mod 32 floats, so the 4 rows x 16 columns
+// of an A operand (lanes l / 16, l % 16) fall in distinct LDS banks
+__host__ __device__ inline int mf_lw(int kw) { return 16 + 32 * ((64 + kw - 1 - 16 + 31) / 32); }
 __host__ __device__ inline int mf_dd(int kh) { return mf_rows(kh) + 15; }
 __host__ inline size_t mfma_lds_bytes(int kmax, int nk) {
   return sizeof(float) * ((size_t)mf_rows(kmax) * mf_lw(kmax) + (size_t)nk * kmax * mf_dd(kmax));

@@ -435,6 +435,8 @@ def main():
     ap_.add_argument("--no-c3", action="store_true")
     ap_.add_argument("--kernel-only", action="store_true",
                      help="run only the roofline kernel (and a calibration copy) for counter collection")
+    ap_.add_argument("--only", choices=["c1", "c2", "c3"],
+                     help="run one side leg alone and print its record (for profiling that leg)")
     args = ap_.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the fits are independent "
@@ -442,6 +444,10 @@ def main():
         return 2
     if args.kernel_only:
         return kernel_only(args)
+    if args.only:
+        leg = {"c1": c1_render_leg, "c2": lambda: c2_leg(args)[0], "c3": c3_leg}[args.only]()
+        print(json.dumps({args.only: leg}))
+        return 0
 
     c4, c4_data = c4_leg(args)
     line = {

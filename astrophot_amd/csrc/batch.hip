@@ -35,6 +35,10 @@ constexpr int NP = APX_BATCH_NP;
 constexpr int kChunk = 256;
 constexpr int kPersist = 2048;
 constexpr int kNChan = 6;  // value + q, PA, n, Re, Ie partials
+// Ie enters the Sersic profile only as the amplitude 10**Ie (sersic_model.py:
+// 61-91), so d model / d Ie = ln(10) model: the Jacobian-stage convolutions
+// skip the Ie plane and write its column from the value (times d Ie / d rep).
+constexpr int kConvChan = kNChan - 1;
 constexpr int kTR = 16;      // output rows of a batch tile (64 x kTR pixels)
 constexpr int kMfmaMax = 35; // widest kernel the MFMA convolution takes
 
@@ -848,13 +852,14 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wu = __builtin_amdgcn_readfirstlane(w);
   constexpr int NKmax = JAC ? 3 : 1;
-  constexpr int nchan = JAC ? kNChan : 1;
+  constexpr int nchan = JAC ? kConvChan : 1;
   constexpr int W = MODE == EPI_RPP ? NP : 1;
   for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
     const int2 e = s.tile_list[k];
     const int g = __builtin_amdgcn_readfirstlane(e.x);
     const apx_batch_galaxy G = b.gal[g];
     const GalState* st = st_of(b, g);
+    const float ie_scale = JAC ? (float)(kLn10 * s.ev[g].dvdr[NP - 1]) : 0.0f;
     const int nxw = G.nx + G.pw + 1, nyw = G.ny + G.ph + 1;  // cutout + 2 * border
     const int ntx = (G.nx + 63) / 64;
     const int ox0 = (e.y % ntx) * 64;
@@ -908,8 +913,12 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
             for (int ww = 0; ww < 4; ++ww) sum += smem[(ww * kRYF + r) * 64 + lane];
             if (!valid[sl]) continue;
             if (JAC) {
-              if (col < 0) out[pp[sl]] = sum;
-              else b.jac[(int64_t)col * b.npix + pp[sl]] = sum;
+              if (col < 0) {
+                out[pp[sl]] = sum;
+                b.jac[(int64_t)(NP - 1) * b.npix + pp[sl]] = ie_scale * sum;
+              } else {
+                b.jac[(int64_t)col * b.npix + pp[sl]] = sum;
+              }
             } else if (MODE == EPI_OUT) {
               out[pp[sl]] = sum;
             } else {
@@ -981,7 +990,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
   extern __shared__ float smem[];
   __shared__ double redd[4][8];  // kConvStaticLds
   constexpr bool JAC = MODE == EPI_NORMAL;
-  constexpr int nchan = JAC ? kNChan : 1;
+  constexpr int nchan = JAC ? kConvChan : 1;
   constexpr int nkt = JAC ? 3 : 1;
   constexpr int W = MODE == EPI_RPP ? NP : 1;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -991,6 +1000,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
     const int g = __builtin_amdgcn_readfirstlane(e.x);
     const apx_batch_galaxy G = b.gal[g];
     const GalState* st = st_of(b, g);
+    const float ie_scale = JAC ? (float)(kLn10 * s.ev[g].dvdr[NP - 1]) : 0.0f;
     const int nxw = G.nx + G.pw + 1, nyw = G.ny + G.ph + 1;  // cutout + 2 * border
     const int ntx = (G.nx + 63) / 64;
     const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kTR;
@@ -1052,8 +1062,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
         for (int i = 0; i < 4; ++i) {
           if (!valid[i]) continue;
           if (JAC) {
-            if (col < 0) out[pp[i]] = acc[i];
-            else b.jac[(int64_t)col * b.npix + pp[i]] = acc[i];
+            if (col < 0) {
+              out[pp[i]] = acc[i];
+              b.jac[(int64_t)(NP - 1) * b.npix + pp[i]] = ie_scale * acc[i];
+            } else {
+              b.jac[(int64_t)col * b.npix + pp[i]] = acc[i];
+            }
           } else if (MODE == EPI_OUT) {
             out[pp[i]] = acc[i];
           } else {

@@ -209,15 +209,16 @@ def timed_events(fn, n=10, skip=2):
 
 def roofline(bf, truth_rep, label):
     """The dominant kernel: the Jacobian-stage convolution (value with K,
-    dK/dcx, dK/dcy and the five partial planes with K: 8 kernel passes per
-    output pixel) of every cutout, timed live; the value convolution beside it."""
+    dK/dcx, dK/dcy and the q, PA, n, Re partial planes with K: 7 kernel passes
+    per output pixel; the Ie column is ln(10) times the model, one multiply)
+    of every cutout, timed live; the value convolution beside it."""
     out = torch.zeros(bf.npix, dtype=torch.float32, device="cuda")
     bf.jacobian(truth_rep, out=out, parts=1)
     jac_us = timed_events(lambda: bf.jacobian(truth_rep, out=out, parts=2))
     bf.render(truth_rep, out=out, parts=1)
     val_us = timed_events(lambda: bf.render(truth_rep, out=out, parts=2))
     vflops = bf.conv_flops()
-    jflops = 8.0 * vflops
+    jflops = 7.0 * vflops
     achieved = jflops / (jac_us * 1e-6)
     kernel = "conv_mfma_kernel<1>" if (bf.b.conv_impl != 1 and bf.b.max_kernel <= 35) else "conv_kernel<1>"
     t = measured_traffic(kernel, len(bf.gal_host))

@@ -7,7 +7,7 @@ run on hand-written HIP kernels behind the C ABI of include/astrophot_hip.h.
 """
 
 from . import AP_config, errors, fit, image, models, param, utils
-from . import batch
+from . import batch, cnn
 
-__all__ = ["AP_config", "errors", "fit", "image", "models", "param", "utils", "batch"]
+__all__ = ["AP_config", "errors", "fit", "image", "models", "param", "utils", "batch", "cnn"]
 __version__ = "0.1.0"

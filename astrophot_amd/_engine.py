@@ -178,9 +178,16 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False, ref_n
             raise SpecificationConflict(f"sampling quad level must be 1..{L.APX_MAX_QUAD_LEVEL}")
         x, w = _gl(cp.sample_quad)
         cp.sq_node[:cp.sample_quad], cp.sq_weight[:cp.sample_quad] = x, w
+    elif sm == "trapezoid":
+        # _model_methods.py:121-139: mean of the four pixel corners; the
+        # reference is that mean plus its Laplacian
+        cp.sampling = L.APX_SAMPLE_TRAPEZOID
+        cp.sample_quad = 2
+        cp.sq_node[:2] = (-1.0, 1.0)
+        cp.sq_weight[:2] = (1.0, 1.0)
     else:
         raise SpecificationConflict(
-            f"{model.name} has unknown sampling mode: {sm}. Should be one of: midpoint, simpsons, quad:level")
+            f"{model.name} has unknown sampling mode: {sm}. Should be one of: midpoint, simpsons, quad:level, trapezoid")
     im = model.integrate_mode if kind != "flat_sky" else "none"
     if im not in ("none", "threshold"):
         raise SpecificationConflict(

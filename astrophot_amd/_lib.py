@@ -22,7 +22,7 @@ APX_SERSIC, APX_EXPONENTIAL, APX_GAUSSIAN, APX_MOFFAT, APX_FLAT_SKY = range(5)
 APX_P_CX, APX_P_CY, APX_P_Q, APX_P_PA, APX_P_A, APX_P_B, APX_P_C = range(7)
 APX_NSLOT = 7
 APX_MAX_CHAN = 8
-APX_SAMPLE_MIDPOINT, APX_SAMPLE_QUAD = 0, 1
+APX_SAMPLE_MIDPOINT, APX_SAMPLE_QUAD, APX_SAMPLE_TRAPEZOID = 0, 1, 2
 APX_MAX_QUAD_LEVEL = 7
 APX_MAX_GRIDDING = 8
 APX_MAX_DEPTH = 3
@@ -99,6 +99,21 @@ SIGNATURES = {
 _lib = None
 
 
+def _apply(lib, signatures) -> None:
+    for name, (res, args) in signatures.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def register(signatures) -> None:
+    """Add ctypes signatures of further entry points; applied at once when the
+    library is already open, so no entry point is ever called untyped."""
+    SIGNATURES.update(signatures)
+    if _lib is not None:
+        _apply(_lib, signatures)
+
+
 def load(require_gpu: bool = True):
     """Open the library (once).  Raises APXError if it is missing, or if
     ``require_gpu`` and torch sees no GPU."""
@@ -109,10 +124,7 @@ def load(require_gpu: bool = True):
         if not LIB_PATH.exists():
             raise APXError(f"{LIB_PATH.name} is not built; run `python -m astrophot_amd._build`")
         lib = ctypes.CDLL(str(LIB_PATH))
-        for name, (res, args) in SIGNATURES.items():
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
+        _apply(lib, SIGNATURES)
         _lib = lib
     return _lib
 

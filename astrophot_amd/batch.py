@@ -74,7 +74,7 @@ SIGNATURES = {
     "apx_batch_start_fallback": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
 }
-L.SIGNATURES.update(SIGNATURES)
+L.register(SIGNATURES)
 
 
 @dataclass

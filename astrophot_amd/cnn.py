@@ -48,7 +48,7 @@ SIGNATURES = {
     "apx_adam": (ctypes.c_int, [_p, _p, _p, _p, _i64, _f32, _f32, _f32, _f32, _i32, _p, _p]),
     "apx_asinh_input": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p]),
 }
-L.SIGNATURES.update(SIGNATURES)
+L.register(SIGNATURES)
 
 
 def kpad(k: int) -> int:

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kInitBlock) void render_init_kernel(apx_component c
     float X, Y;
     pixel_xy(p, cp, i, j, X, Y);
     float g[APX_NSLOT];
-    if (cp.sampling == APX_SAMPLE_QUAD) {
+    if (cp.sampling != APX_SAMPLE_MIDPOINT) {
       float centre = 0.0f;
       value = quad_sample<float, GRAD>(p, cp, X, Y, cp.sample_quad, cp.sq_node, cp.sq_weight, 1.0f,
                                        0.5f * p.P00, 0.5f * p.P01, 0.5f * p.P10, 0.5f * p.P11, g,
@@ -133,10 +133,18 @@ __device__ bool select_f64(const apx_component& cp, const Prof<double>& p, int i
                                                   0.5 * p.P10, 0.5 * p.P11, nullptr, centre);
     return fabs(res - centre) > thr;
   }
+  // midpoint: Laplacian of the pixel-centre values; trapezoid: of the means of
+  // the pixel corners (_model_methods.py:121-139)
   const int ii = min(max(i, 1), cp.nx - 2), jj = min(max(j, 1), cp.ny - 2);
   double X, Y;
   auto at = [&](int a, int b) {
     pixel_xy(p, cp, a, b, X, Y);
+    if (cp.sampling == APX_SAMPLE_TRAPEZOID) {
+      double unused;
+      return quad_sample<double, false>(p, cp, X, Y, cp.sample_quad, cp.sq_node, cp.sq_weight,
+                                        1.0, 0.5 * p.P00, 0.5 * p.P01, 0.5 * p.P10, 0.5 * p.P11,
+                                        nullptr, unused);
+    }
     return eval_profile<double, false>(p, X, Y, 1.0, nullptr);
   };
   const double mc = at(ii, jj);
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(256) void render_vjp_kernel(apx_component cp,
     const int i = (int)(pix % cp.nx), j = (int)(pix / cp.nx);
     float X, Y, g[APX_NSLOT];
     pixel_xy(p, cp, i, j, X, Y);
-    if (cp.sampling == APX_SAMPLE_QUAD) {
+    if (cp.sampling != APX_SAMPLE_MIDPOINT) {
       float centre;
       quad_sample<float, true>(p, cp, X, Y, cp.sample_quad, cp.sq_node, cp.sq_weight, 1.0f,
                                0.5f * p.P00, 0.5f * p.P01, 0.5f * p.P10, 0.5f * p.P11, g, centre);
@@ -369,7 +377,9 @@ int render_component(const apx_component* cp_in, float* planes, void* workspace,
   for (int c = 1; c < cp.nchan; ++c)
     if (cp.chan_slot[c] < 0 || cp.chan_slot[c] >= APX_NSLOT)
       return set_error(APX_E_INVALID, "render: bad channel slot");
-  if (cp.sampling == APX_SAMPLE_QUAD &&
+  if (cp.sampling < APX_SAMPLE_MIDPOINT || cp.sampling > APX_SAMPLE_TRAPEZOID)
+    return set_error(APX_E_INVALID, "render: unknown sampling mode");
+  if (cp.sampling != APX_SAMPLE_MIDPOINT &&
       (cp.sample_quad < 1 || cp.sample_quad > APX_MAX_QUAD_LEVEL))
     return set_error(APX_E_LIMIT, "render: sampling quad level out of range");
   if (cp.integrate) {

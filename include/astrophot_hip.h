@@ -57,6 +57,7 @@ extern "C" {
 
 #define APX_SAMPLE_MIDPOINT 0
 #define APX_SAMPLE_QUAD 1
+#define APX_SAMPLE_TRAPEZOID 2
 
 #define APX_MAX_QUAD_LEVEL 7 /* quad_level**2 <= 64 lanes */
 #define APX_MAX_GRIDDING 8   /* gridding**2 <= 64 lanes */
@@ -69,10 +70,14 @@ extern "C" {
  * (X, Y) = P . (i - ci, j - cj) + frac from the model centre. */
 typedef struct apx_component {
   int32_t profile;        /* APX_SERSIC .. APX_FLAT_SKY */
-  int32_t sampling;       /* APX_SAMPLE_MIDPOINT | APX_SAMPLE_QUAD */
+  int32_t sampling;       /* APX_SAMPLE_MIDPOINT | APX_SAMPLE_QUAD | APX_SAMPLE_TRAPEZOID */
   int32_t sample_quad;    /* N of sampling_mode "quad:N"; "simpsons" (_model_methods.py:
                              97-109) is N = 3 with sq_node -1, 0, 1 and sq_weight
-                             1/3, 4/3, 1/3: the 3 x 3 Simpson rule on the pixel */
+                             1/3, 4/3, 1/3: the 3 x 3 Simpson rule on the pixel.
+                             APX_SAMPLE_TRAPEZOID ("trapezoid", _model_methods.py:
+                             121-139) takes N = 2 with sq_node -1, 1 and sq_weight
+                             1, 1 (the mean of the four pixel corners) and selects by
+                             the Laplacian of those means, as midpoint sampling does */
   int32_t integrate;      /* 0 = integrate_mode "none", 1 = "threshold" */
   int32_t quad_level;     /* integrate_quad_level */
   int32_t gridding;       /* integrate_gridding */

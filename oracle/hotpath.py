@@ -35,6 +35,10 @@ component_sample           flux conservation, whole-pixel translation, q=1
                            PA-independence, symmetry invariants
 simpsons sampling          exact on cubics (4-point Gauss-Legendre of the
                            same parallelogram pixel); centre sample = value
+trapezoid sampling         closed form: a round Gaussian is separable, so the
+                           corner mean is the product of 1-D endpoint means;
+                           exact on bilinear functions of a sheared pixel;
+                           reference = brute-force 5-point Laplacian
 psf_sample                 60 x 60 brute-force pixel integral of the Moffat
                            PSF, unit sum, transpose/flip symmetry
 auxiliary PSF model        equals the galaxy convolved with the sampled PSF
@@ -214,6 +218,16 @@ class Grid:
         midpoints and centre, (2, 2 ny + 1, 2 nx + 1)."""
         xs = 0.5 * torch.arange(2 * self.nx + 1, dtype=F64) - 0.5
         ys = 0.5 * torch.arange(2 * self.ny + 1, dtype=F64) - 0.5
+        mi, mj = torch.meshgrid(xs, ys, indexing="xy")
+        c = self.P @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.ref_ij.view(2, 1))
+        c = c + self.ref_xy.view(2, 1)
+        return c.view(2, *mi.shape)
+
+    def corner_meshgrid(self):
+        """window_object.py:408-426: plane coords of every pixel corner,
+        (2, ny + 1, nx + 1)."""
+        xs = torch.arange(self.nx + 1, dtype=F64) - 0.5
+        ys = torch.arange(self.ny + 1, dtype=F64) - 0.5
         mi, mj = torch.meshgrid(xs, ys, indexing="xy")
         c = self.P @ (torch.stack((mi.reshape(-1), mj.reshape(-1))) - self.ref_ij.view(2, 1))
         c = c + self.ref_xy.view(2, 1)
@@ -531,7 +545,8 @@ class Component:
 
 
 def sample_init(comp, grid, v, center):
-    """_model_methods.py:82-148 (midpoint and quad:N)."""
+    """_model_methods.py:82-148: returns (reference, deep) for the midpoint,
+    simpsons, quad:N and trapezoid sampling modes."""
     Coords = grid.meshgrid()
     X, Y = Coords[0] - center[0], Coords[1] - center[1]
     ev = lambda x, y, P: comp.evaluate(x, y, P, v)
@@ -548,6 +563,11 @@ def sample_init(comp, grid, v, center):
         mid = dens[1::2, 1::2]
         simps = F.conv2d(dens.view(1, 1, *dens.shape), SIMPSONS.view(1, 1, 3, 3), stride=2, padding="valid")
         return mid, simps.squeeze(0).squeeze(0)
+    if comp.sampling_mode == "trapezoid":  # _model_methods.py:121-139
+        Cc = grid.corner_meshgrid()
+        dens = ev(Cc[0] - center[0], Cc[1] - center[1], grid.P)
+        trapz = 0.25 * (dens[:-1, :-1] + dens[:-1, 1:] + dens[1:, :-1] + dens[1:, 1:])
+        return trapz + curvature(trapz), trapz
     raise ValueError(f"unknown sampling mode {comp.sampling_mode}")
 
 

@@ -64,6 +64,32 @@ def test_sersic_quad_sampling():
     assert rel_err(m().data, ref) < IMG_TOL
 
 
+@pytest.mark.parametrize("kind", ["sersic", "exponential", "gaussian", "moffat"])
+@pytest.mark.parametrize("integrate", ["none", "threshold"])
+def test_trapezoid_sampling(kind, integrate):
+    """sampling_mode "trapezoid" (_model_methods.py:121-139): the mean of the
+    four pixel corners, selection by its Laplacian.  Image, Jacobian columns
+    and the adjoint against the oracle, with the PSF convolved."""
+    om, m = single(kind, GAL[kind], psf=PSF11, psf_mode="full", sampling_mode="trapezoid",
+                   integrate_mode=integrate)
+    rep = om.vector_representation()
+    assert rel_err(m().data, om.forward(rep)) < IMG_TOL
+    Jref = om.jacobian(rep)
+    J = m.jacobian(parameters=rep.clone(), as_representation=True).data
+    for k in range(J.shape[-1]):
+        assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k
+    G = torch.tensor(np.random.default_rng(3).normal(size=(m.window.ny, m.window.nx)))
+    r = rep.clone().requires_grad_(True)
+    ref, = torch.autograd.grad((om.forward(r) * G).sum(), r)
+    got = m.vjp(G.float(), parameters=rep.clone(), as_representation=True).cpu()
+    assert ((got - ref).abs() / ref.abs().max()).max() < JAC_TOL, (got, ref)
+
+
+def test_trapezoid_sampling_psf_none():
+    om, m = single("sersic", GAL["sersic"], sampling_mode="trapezoid")
+    assert rel_err(m().data, om.forward(om.vector_representation())) < IMG_TOL
+
+
 def test_group_c0_like():
     """Sersic + flat sky group with a 25x25 Gaussian PSF (the 100x100 C0 setup at 64x64)."""
     nx = ny = 64

@@ -9,6 +9,7 @@ import re
 import numpy as np
 import pytest
 import torch
+from scipy.special import roots_legendre
 
 import astrophot_amd as ap
 from astrophot_amd import _lib as L
@@ -209,3 +210,41 @@ def test_engine_descriptors(monkeypatch):
     assert shift[4] == pytest.approx(float(pc[1] - torch.round(pc[1])))
     conv = [a for n, a in fake.calls if n == "apx_convolve"][0][0]._obj
     assert (conv.x0, conv.y0, conv.ow, conv.oh) == (6, 6, 40, 36)
+
+
+@pytest.mark.parametrize("mode,code,n,nodes,weights", [
+    ("midpoint", 0, None, None, None),
+    ("simpsons", 1, 3, [-1.0, 0.0, 1.0], [1 / 3, 4 / 3, 1 / 3]),
+    ("quad:4", 1, 4, list(roots_legendre(4)[0]), list(roots_legendre(4)[1])),
+    ("trapezoid", 2, 2, [-1.0, 1.0], [1.0, 1.0]),
+])
+def test_sampling_mode_descriptors(monkeypatch, mode, code, n, nodes, weights):
+    """_model_methods.py:82-148: each sampling mode's rule as the render
+    kernel receives it; the weights w_u w_v / 4 sum to one on every pixel."""
+    fake = _FakeLib()
+    monkeypatch.setattr(L, "load", lambda require_gpu=True: fake)
+    monkeypatch.setattr(L, "stream", lambda: 0)
+    tgt = ap.image.Target_Image(data=np.zeros((20, 24)), pixelscale=1.0)
+    m = ap.models.AstroPhot_Model(name="s", model_type="sersic galaxy model", target=tgt, sampling_mode=mode,
+                                  parameters={"center": [12.3, 9.1], "q": 0.6, "PA": 0.7, "n": 2.5, "Re": 4.0,
+                                              "Ie": 1.0})
+    m()
+    comp = [a for nm, a in fake.calls if nm == "apx_render_component"][0][0]._obj
+    assert comp.sampling == code
+    if n is not None:
+        assert comp.sample_quad == n
+        assert np.allclose(list(comp.sq_node)[:n], nodes, rtol=0, atol=1e-15)
+        assert np.allclose(list(comp.sq_weight)[:n], weights, rtol=0, atol=1e-15)
+        assert sum(weights) ** 2 / 4 == pytest.approx(1.0, abs=1e-14)
+
+
+def test_unknown_sampling_mode_raises(monkeypatch):
+    fake = _FakeLib()
+    monkeypatch.setattr(L, "load", lambda require_gpu=True: fake)
+    monkeypatch.setattr(L, "stream", lambda: 0)
+    tgt = ap.image.Target_Image(data=np.zeros((20, 24)), pixelscale=1.0)
+    m = ap.models.AstroPhot_Model(name="s", model_type="sersic galaxy model", target=tgt, sampling_mode="boole",
+                                  parameters={"center": [12.3, 9.1], "q": 0.6, "PA": 0.7, "n": 2.5, "Re": 4.0,
+                                              "Ie": 1.0})
+    with pytest.raises(ap.errors.SpecificationConflict, match="trapezoid"):
+        m()

@@ -297,6 +297,68 @@ def test_simpsons_sampling_exact_on_cubics(P):
         assert torch.allclose(mid, comp.evaluate(C[0], C[1], Pm, None), rtol=1e-14, atol=1e-14)
 
 
+def _laplacian_replicate(a):
+    """5-point Laplacian on the interior, border rows/columns copied outward."""
+    ny, nx = a.shape
+    L = torch.zeros_like(a)
+    for j in range(1, ny - 1):
+        for i in range(1, nx - 1):
+            L[j, i] = a[j, i - 1] + a[j, i + 1] + a[j - 1, i] + a[j + 1, i] - 4 * a[j, i]
+    L[0, :], L[-1, :] = L[1, :], L[-2, :]
+    L[:, 0], L[:, -1] = L[:, 1], L[:, -2]
+    return L
+
+
+def test_trapezoid_sampling_round_gaussian_closed_form():
+    """_model_methods.py:121-139.  A round Gaussian is separable in the plane
+    axes, so with a diagonal pixelscale the mean of a pixel's four corner
+    values is the product of the means of the 1-D profile at the pixel's two
+    x edges and two y edges.  The returned reference is that mean plus its
+    Laplacian (replicate border)."""
+    P = [[0.8, 0.0], [0.0, 1.1]]
+    grid = H.Grid(9, 7, P, ref_ij=(0.7, -0.2), ref_xy=(0.4, 0.1))
+    c = torch.tensor([3.3, 2.1], dtype=F64)
+    sigma, flux = 1.7, 0.3
+    comp = H.Component("gaussian", grid, {"center": c.tolist(), "q": 1.0, "PA": 0.4, "sigma": sigma,
+                                          "flux": flux}, sampling_mode="trapezoid")
+    v = {"center": c, "q": torch.tensor(1.0, dtype=F64), "PA": torch.tensor(0.4, dtype=F64),
+         "sigma": torch.tensor(sigma, dtype=F64), "flux": torch.tensor(flux, dtype=F64)}
+    ref, trapz = H.sample_init(comp, grid, v, c)
+    xe = 0.8 * (np.arange(10) - 0.5 - 0.7) + 0.4 - 3.3
+    ye = 1.1 * (np.arange(8) - 0.5 + 0.2) + 0.1 - 2.1
+    ex, ey = np.exp(-xe ** 2 / (2 * sigma ** 2)), np.exp(-ye ** 2 / (2 * sigma ** 2))
+    C = 0.8 * 1.1 * 10 ** flux / math.sqrt(2 * math.pi * sigma ** 2) * math.exp(-1e-6 / (2 * sigma ** 2))
+    closed = C * np.outer(0.5 * (ey[:-1] + ey[1:]), 0.5 * (ex[:-1] + ex[1:]))
+    assert np.allclose(trapz.numpy(), closed, rtol=1e-13, atol=0)
+    assert torch.allclose(ref, trapz + _laplacian_replicate(trapz), rtol=1e-13, atol=1e-16)
+
+
+class _PixelBilinear:
+    """Stand-in component whose brightness is (a + b u)(c + d v) in the pixel
+    coordinates (u, v) of a sheared grid."""
+
+    sampling_mode = "trapezoid"
+
+    def __init__(self, grid):
+        self.Pinv = torch.linalg.inv(grid.P)
+
+    def evaluate(self, X, Y, P, v):
+        u = self.Pinv[0, 0] * X + self.Pinv[0, 1] * Y
+        w = self.Pinv[1, 0] * X + self.Pinv[1, 1] * Y
+        return (0.7 + 0.3 * u) * (1.2 - 0.4 * w)
+
+
+def test_trapezoid_sampling_exact_on_pixel_bilinear():
+    """The corner mean is the pixel average of a function bilinear in the
+    pixel coordinates: for (a + b u)(c + d v) that average is its value at the
+    pixel centre, whatever the shear of the pixelscale."""
+    grid = H.Grid(6, 5, [[0.7, 0.2], [-0.1, 0.9]], ref_ij=(1.3, 0.4), ref_xy=(0.2, -0.7))
+    comp = _PixelBilinear(grid)
+    _, trapz = H.sample_init(comp, grid, {}, torch.zeros(2, dtype=F64))
+    Cm = grid.meshgrid()
+    assert torch.allclose(trapz, comp.evaluate(Cm[0], Cm[1], grid.P, None), rtol=1e-13, atol=1e-13)
+
+
 def _psf_comp(n=2.5, Rd=2.2, side=15, P=1.0, **kw):
     g = H.Grid(side, side, P, ref_ij=((side - 1) / 2, (side - 1) / 2), ref_xy=(0.0, 0.0))
     kw.setdefault("sampling_mode", "simpsons")

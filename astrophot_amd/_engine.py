@@ -147,6 +147,8 @@ class Plan:
     shift: Optional[Tuple[float, float]]
     psf_full: bool
     chan_slots: List[int]
+    upscale: int = 1           # PSF upscale: working pixels per image pixel, per axis
+    out_window: object = None  # where the convolved output lands (after the block sum)
 
 
 def plan(model, values, window, slot_cols=None, dvdr=None, channels=False, ref_npix=None) -> Plan:
@@ -203,12 +205,21 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False, ref_n
         cp.gl_node[:cp.quad_level], cp.gl_weight[:cp.quad_level] = x, w
 
     shift = None
+    upscale = 1
+    out_window = window.copy()
     if psf_full:
         psf = model.psf
         if psf is None:
             raise SpecificationConflict(f"{model.name} has psf_mode 'full' but no PSF")
-        if round(math.sqrt(window._area / psf.window._area)) != 1:
-            raise SpecificationConflict("PSF pixelscale must equal the image pixelscale (upscale 1)")
+        upscale = psf_upscale(window, psf)
+        if upscale != 1:
+            # model_object.py:313-314: 1 / psf_upscale of an int32 tensor is a
+            # float32 scalar, so the working pixelscale is P * float32(1/U) and
+            # the working shape is the floor of the exact quotient n / float32(1/U)
+            window = window.rescale_pixel(float(np.float32(1.0 / upscale)))
+            # model_object.py:348-349: the block sum lands on that grid rescaled by U
+            out_window = window.rescale_pixel(upscale)
+            P = window._P
         bx, by = psf.psf_border_int
         window.pad_pixel((bx, by))
         pcx, pcy = window._x2p(cx, cy)
@@ -262,7 +273,15 @@ def plan(model, values, window, slot_cols=None, dvdr=None, channels=False, ref_n
     for s, v in (dvdr or {}).items():
         if s < L.APX_NSLOT:  # an auxiliary PSF model's slots are not this render's
             cp.dvdr[s] = float(v)
-    return Plan(cp, window, bx, by, shift, psf_full, chan_slots)
+    return Plan(cp, window, bx, by, shift, psf_full, chan_slots, upscale, out_window)
+
+
+def psf_upscale(window, psf) -> int:
+    """model_object.py:313: round(window pixel length / PSF pixel length)."""
+    U = round(float(window.pixel_length) / float(psf.window.pixel_length))
+    if U < 1:
+        raise SpecificationConflict("PSF pixels must not be larger than the image pixels")
+    return U
 
 
 @dataclass
@@ -373,6 +392,8 @@ def _sampled_part(model, req: RenderRequest):
     w = req.window
     if region is None or model._profile != "sersic" or getattr(model, "_is_psf", False) or req.jac is not None:
         return w, None
+    if "full" in model.psf_mode and model.psf is not None and psf_upscale(w, model.psf) != 1:
+        return w, None
     part = region & w
     if part.nx * part.ny >= w.nx * w.ny:
         return w, None
@@ -414,52 +435,76 @@ def render(model, values: Dict[str, torch.Tensor], req: RenderRequest):
     cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
     cws = _buffer(model, "conv_ws", cwsb, dev)
     job = _new_job(pl, kh, kw)
-    bx, by = pl.bx, pl.by
+    bx, by, U, out_w = pl.bx, pl.by, pl.upscale, pl.out_window
     if req.image is not None:
         # default: the part of the image the sampling window covers
-        region = req.region if req.region is not None else (req.image.window & sample_window)
-        (dy, dx), (sy, sx), (h, w) = _box(req.image.window, sample_window, region)
+        region = req.region if req.region is not None else (req.image.window & out_w)
+        (dy, dx), (sy, sx), (h, w) = _box(req.image.window, out_w, region)
         img = req.image.data
         if img.dtype != torch.float32 or not img.is_contiguous():
             raise SpecificationConflict("model image must be contiguous float32")
         j1 = _job_copy(job)
         j1.nchan = 1
-        j1.x0, j1.y0, j1.ow, j1.oh = bx + sx, by + sy, w, h
-        j1.img, j1.img_row, j1.img_x0, j1.img_y0 = L.ptr(img), img.shape[1], dx, dy
-        j1.img_w, j1.img_h = img.shape[1], img.shape[0]
+        j1.x0, j1.y0, j1.ow, j1.oh = bx + U * sx, by + U * sy, U * w, U * h
+        if U == 1:
+            j1.img, j1.img_row, j1.img_x0, j1.img_y0 = L.ptr(img), img.shape[1], dx, dy
+            j1.img_w, j1.img_h = img.shape[1], img.shape[0]
+        elif h > 0 and w > 0:
+            fine = _fine(model, "fine_img", (U * h, U * w, 1), dev)
+            j1.img, j1.img_row, j1.img_x0, j1.img_y0, j1.img_w, j1.img_h = L.ptr(fine), U * w, 0, 0, U * w, U * h
         if h > 0 and w > 0:
-            _launch("convolve", 2.0 * kh * kw * h * w, lambda: L.check(
+            _launch("convolve", 2.0 * kh * kw * h * w * U * U, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j1), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
+            if U != 1:
+                _block_sum(lib, fine, U, (h, w), img, 1, (dx, dy), [0], st)
     if req.jac is not None and (slot_cols or psf_cols):
         # default: the part of the Jacobian the sampling window covers (a
         # component on a sub-window of a group's Jacobian lands at its own pixels)
-        region = req.region if req.region is not None else (req.jac.window & sample_window)
-        (dy, dx), (sy, sx), (h, w) = _box(req.jac.window, sample_window, region)
+        region = req.region if req.region is not None else (req.jac.window & out_w)
+        (dy, dx), (sy, sx), (h, w) = _box(req.jac.window, out_w, region)
         jd = req.jac.data
         if not jd.is_contiguous() or jd.dtype != torch.float32:
             raise SpecificationConflict("Jacobian must be contiguous float32")
         npj = int(jd.shape[2])
+        centre_dk = pl.psf_full and pl.shift is not None
+        dest = [slot_cols[s] for s in pl.chan_slots]
+        if centre_dk:
+            dest += [slot_cols[s] for s in (L.APX_P_CX, L.APX_P_CY) if s in slot_cols]
+        dest += [psf_cols[s] for s in sorted(psf_cols)]
+        # the columns this render writes: in place, or through a fine-grid
+        # buffer holding just them when the PSF is upscaled
+        col = {c: c for c in dest} if U == 1 else {c: k for k, c in enumerate(dest)}
+        col[-1] = -1
         j2 = _job_copy(job)
-        j2.x0, j2.y0, j2.ow, j2.oh = bx + sx, by + sy, w, h
-        j2.jac, j2.jac_row, j2.jac_np, j2.jac_x0, j2.jac_y0 = L.ptr(jd), jd.shape[1] * npj, npj, dx, dy
-        j2.jac_w, j2.jac_h = jd.shape[1], jd.shape[0]
+        j2.x0, j2.y0, j2.ow, j2.oh = bx + U * sx, by + U * sy, U * w, U * h
+        if U == 1:
+            j2.jac, j2.jac_row, j2.jac_np, j2.jac_x0, j2.jac_y0 = L.ptr(jd), jd.shape[1] * npj, npj, dx, dy
+            j2.jac_w, j2.jac_h = jd.shape[1], jd.shape[0]
+        elif h > 0 and w > 0:
+            if len(dest) > L.APX_MAX_BLOCK_COLS:
+                raise SpecificationConflict(f"an upscaled render writes at most {L.APX_MAX_BLOCK_COLS} columns")
+            fine = _fine(model, "fine_jac", (U * h, U * w, len(dest)), dev)
+            j2.jac, j2.jac_row, j2.jac_np, j2.jac_x0, j2.jac_y0 = L.ptr(fine), U * w * len(dest), len(dest), 0, 0
+            j2.jac_w, j2.jac_h = U * w, U * h
         for c, s in enumerate(pl.chan_slots):
-            j2.chan_col[c + 1] = slot_cols[s]
-        if pl.psf_full and pl.shift is not None:
-            j2.dk_col[0] = slot_cols.get(L.APX_P_CX, -1)
-            j2.dk_col[1] = slot_cols.get(L.APX_P_CY, -1)
+            j2.chan_col[c + 1] = col[slot_cols[s]]
+        if centre_dk:
+            j2.dk_col[0] = col[slot_cols.get(L.APX_P_CX, -1)]
+            j2.dk_col[1] = col[slot_cols.get(L.APX_P_CY, -1)]
         nk = (cp.nchan - 1) + (2 if j2.dk_col[0] >= 0 or j2.dk_col[1] >= 0 else 0)
         if h > 0 and w > 0 and nk > 0:
-            _launch("convolve_jac", 2.0 * kh * kw * h * w * nk, lambda: L.check(
+            _launch("convolve_jac", 2.0 * kh * kw * h * w * nk * U * U, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j2), L.ptr(planes), kptr, L.ptr(cws), cwsb, st)))
         # auxiliary PSF parameters: conv(value plane, dK/dtheta), two columns a launch
         for ptr, (c0, c1) in (_dk_pairs(kern, psf_cols) if h > 0 and w > 0 else ()):
             j3 = _job_copy(j2)
             j3.nchan = 1
             j3.chan_col[:] = [-1] * L.APX_MAX_CHAN
-            j3.dk_col[0], j3.dk_col[1] = c0, c1
-            _launch("convolve_jac", 2.0 * kh * kw * h * w * 2, lambda: L.check(
+            j3.dk_col[0], j3.dk_col[1] = col[c0], col[c1]
+            _launch("convolve_jac", 2.0 * kh * kw * h * w * 2 * U * U, lambda: L.check(
                 lib.apx_convolve(ctypes.byref(j3), L.ptr(planes), ptr, L.ptr(cws), cwsb, st)))
+        if U != 1 and h > 0 and w > 0:
+            _block_sum(lib, fine, U, (h, w), jd, npj, (dx, dy), dest, st)
     return ws
 
 
@@ -481,15 +526,21 @@ def vjp(model, values, window, G, G_window, slot_cols, dvdr, region=None):
     slot_cols = {s: c for s, c in all_cols.items() if s < PSF_SLOT0}
     psf_cols = {s - PSF_SLOT0: c for s, c in all_cols.items() if s >= PSF_SLOT0}
     pl = plan(model, values, window, slot_cols, dvdr, channels=True)
-    cp, nx, ny, bx, by = pl.cp, pl.cp.nx, pl.cp.ny, pl.bx, pl.by
+    cp, nx, ny, bx, by, U = pl.cp, pl.cp.nx, pl.cp.ny, pl.bx, pl.by, pl.upscale
     region = G_window if region is None else region
-    (dy, dx), (sy, sx), (h, w) = _box(G_window, window, region)
+    (dy, dx), (sy, sx), (h, w) = _box(G_window, pl.out_window, region)
     out = torch.zeros(len(all_cols), dtype=torch.float64, device=dev)
     col_index = {c: k for k, c in enumerate(sorted(all_cols.values()))}
     if h <= 0 or w <= 0:
         return out, col_index
+    # the adjoint of the U x U block sum gives every fine pixel of a block its
+    # coarse pixel's G
+    Gc = G[dy:dy + h, dx:dx + w]
+    if U != 1:
+        Gc = Gc.repeat_interleave(U, dim=0).repeat_interleave(U, dim=1)
+    sx, sy, h, w = U * sx, U * sy, U * h, U * w
     Gw = torch.zeros((ny, nx), dtype=torch.float32, device=dev)
-    Gw[by + sy:by + sy + h, bx + sx:bx + sx + w] = G[dy:dy + h, dx:dx + w]
+    Gw[by + sy:by + sy + h, bx + sx:bx + sx + w] = Gc
     kern = _kernels(model, pl, dvdr, dev, st, sorted(psf_cols))
     kh, kw = kern.kh, kern.kw
     cwsb = lib.apx_convolve_workspace_bytes(kh, kw)
@@ -524,7 +575,7 @@ def vjp(model, values, window, G, G_window, slot_cols, dvdr, region=None):
     if psf_dk:
         pairs.extend(_dk_pairs(kern, psf_cols))
     if pairs:
-        Gb = G[dy:dy + h, dx:dx + w].double()
+        Gb = Gc.double()
         C = torch.empty((h, w, 2), dtype=torch.float32, device=dev)
         for ptr, cols in pairs:
             C.zero_()
@@ -538,6 +589,29 @@ def vjp(model, values, window, G, G_window, slot_cols, dvdr, region=None):
                 if cols[k] >= 0:
                     out[col_index[cols[k]]] += (Gb * C[..., k].double()).sum()
     return out, col_index
+
+
+def _fine(model, name, shape, dev):
+    """A zeroed float32 buffer of ``shape`` on the fine (PSF-upscaled) grid."""
+    n = shape[0] * shape[1] * shape[2]
+    buf = _buffer(model, name, 4 * n, dev)[:4 * n].view(torch.float32).view(*shape)
+    buf.zero_()
+    return buf
+
+
+def _block_sum(lib, fine, U, hw, dst, dst_pix, dxy, cols, st):
+    """Sum the U x U blocks of ``fine`` (fine rows x fine cols x len(cols))
+    into ``dst`` (rows x cols x dst_pix, or rows x cols) at pixel ``dxy``,
+    channel k into offset cols[k] (image_object.py:348-376, 450-457)."""
+    job = L.BlockSum()
+    job.scale, job.nc, job.h, job.w = U, len(cols), hw[0], hw[1]
+    job.src_h, job.src_w = int(fine.shape[0]), int(fine.shape[1])
+    job.dst, job.dst_pix = L.ptr(dst), dst_pix
+    job.dst_h, job.dst_w = int(dst.shape[0]), int(dst.shape[1])
+    job.dst_row = int(dst.shape[1]) * dst_pix
+    job.dst_x0, job.dst_y0 = dxy
+    job.col[:len(cols)] = list(cols)
+    L.check(lib.apx_block_sum_add(ctypes.byref(job), L.ptr(fine), st))
 
 
 def _job_copy(job):

@@ -59,6 +59,19 @@ class ConvJob(ctypes.Structure):
     ]
 
 
+APX_MAX_BLOCK_COLS = 16
+
+
+class BlockSum(ctypes.Structure):
+    """apx_block_sum."""
+
+    _fields_ = [
+        ("scale", _i32), ("nc", _i32), ("w", _i32), ("h", _i32), ("src_w", _i32), ("src_h", _i32),
+        ("dst", _p), ("dst_row", _i64), ("dst_pix", _i32), ("dst_w", _i32), ("dst_h", _i32),
+        ("dst_x0", _i32), ("dst_y0", _i32), ("col", _i32 * APX_MAX_BLOCK_COLS),
+    ]
+
+
 class Group(ctypes.Structure):
     """apx_group."""
 
@@ -89,6 +102,7 @@ SIGNATURES = {
     "apx_psf_kernels_dparam": (ctypes.c_int, [_p, _p, _i32, _i32, _i32, _i32, _f64, _f64, _p, _p]),
     "apx_convolve_workspace_bytes": (_i64, [_i32, _i32]),
     "apx_convolve": (ctypes.c_int, [ctypes.POINTER(ConvJob), _p, _p, _p, _i64, _p]),
+    "apx_block_sum_add": (ctypes.c_int, [ctypes.POINTER(BlockSum), _p, _p]),
     "apx_reduce_workspace_bytes": (_i64, [_i64, _i32]),
     "apx_chi2": (ctypes.c_int, [_p, _p, _p, _p, _i64, _p, _p, _i64, _p]),
     "apx_normal_equations": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _i32, _p, _p, _p, _i64, _p]),

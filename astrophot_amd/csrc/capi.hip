@@ -79,6 +79,10 @@ int apx_convolve(const apx_conv_job* job, const float* planes, const float* kern
                        static_cast<hipStream_t>(stream));
 }
 
+int apx_block_sum_add(const apx_block_sum* job, const float* src, void* stream) {
+  return apx::block_sum_add(job, src, static_cast<hipStream_t>(stream));
+}
+
 int64_t apx_group_workspace_bytes(const apx_group* group) { return apx::group_workspace_bytes(group); }
 
 int apx_group_image(const apx_group* group, const float* src_model, double sky, float* out, void* stream) {

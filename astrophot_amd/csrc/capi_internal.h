@@ -27,6 +27,7 @@ int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, co
               double dvdr_x, double dvdr_y, float* kernels, hipStream_t stream);
 int psf_kernels_dparam(const float* psf, const float* dpsf, int32_t nd, int32_t ph, int32_t pw,
                        int32_t shifted, double sx, double sy, float* out, hipStream_t stream);
+int block_sum_add(const apx_block_sum* job, const float* src, hipStream_t stream);
 int64_t convolve_workspace_bytes(int32_t kh, int32_t kw);
 int convolve(const apx_conv_job* job, const float* planes, const float* kernels, void* workspace,
              int64_t ws_bytes, hipStream_t stream);

@@ -306,6 +306,47 @@ int psf_kernels_dparam(const float* psf, const float* dpsf, int32_t nd, int32_t 
   return check_launch("psf_kernels_dparam");
 }
 
+// Block sums of an upscaled render back onto the image grid: one thread per
+// (coarse pixel, channel), src read with a stride of nc floats.
+__global__ __launch_bounds__(256) void block_sum_add_kernel(apx_block_sum job,
+                                                            const float* __restrict__ src) {
+  const int64_t n = (int64_t)job.w * job.h * job.nc;
+  const int U = job.scale;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int c = (int)(e % job.nc);
+    const int64_t p = e / job.nc;
+    const int x = (int)(p % job.w), y = (int)(p / job.w);
+    float s = 0.0f;
+    for (int b = 0; b < U; ++b) {
+      const float* row = src + ((int64_t)(U * y + b) * job.src_w + (int64_t)U * x) * job.nc + c;
+      for (int a = 0; a < U; ++a) s += row[(int64_t)a * job.nc];
+    }
+    job.dst[(int64_t)(job.dst_y0 + y) * job.dst_row + (int64_t)(job.dst_x0 + x) * job.dst_pix +
+            job.col[c]] += s;
+  }
+}
+
+int block_sum_add(const apx_block_sum* jp, const float* src, hipStream_t stream) {
+  if (!jp || !src || !jp->dst) return set_error(APX_E_INVALID, "block_sum_add: null pointer");
+  const apx_block_sum job = *jp;
+  if (job.scale < 1 || job.nc < 1 || job.nc > APX_MAX_BLOCK_COLS)
+    return set_error(APX_E_INVALID, "block_sum_add: scale or nc out of range");
+  if (job.w < 0 || job.h < 0) return set_error(APX_E_INVALID, "block_sum_add: negative box");
+  if (job.w == 0 || job.h == 0) return APX_OK;
+  if ((int64_t)job.scale * job.w > job.src_w || (int64_t)job.scale * job.h > job.src_h)
+    return set_error(APX_E_INVALID, "block_sum_add: box outside the source");
+  if (job.dst_pix < 1 || job.dst_row < (int64_t)job.dst_w * job.dst_pix || job.dst_x0 < 0 ||
+      job.dst_y0 < 0 || job.dst_x0 + job.w > job.dst_w || job.dst_y0 + job.h > job.dst_h)
+    return set_error(APX_E_INVALID, "block_sum_add: box outside the destination");
+  for (int c = 0; c < job.nc; ++c)
+    if (job.col[c] < 0 || job.col[c] >= job.dst_pix)
+      return set_error(APX_E_INVALID, "block_sum_add: column out of range");
+  const int64_t n = (int64_t)job.w * job.h * job.nc;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(block_sum_add_kernel, dim3(blocks), dim3(256), 0, stream, job, src);
+  return check_launch("block_sum_add");
+}
+
 int64_t convolve_workspace_bytes(int32_t kh, int32_t kw) {
   if (kh < 1 || kw < 1) return 0;
   return (int64_t)sizeof(float) * 3 * kw * fast_kts(kh) + 256;

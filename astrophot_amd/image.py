@@ -593,7 +593,7 @@ class Jacobian_Image(Image):
 class Target_Image(Image):
     """image/target_image.py:17-532: data, variance/weight, mask and PSF."""
 
-    def __init__(self, *args, mask=None, variance=None, weight=None, psf=None, **kwargs):
+    def __init__(self, *args, mask=None, variance=None, weight=None, psf=None, psf_upscale=1, **kwargs):
         super().__init__(*args, **kwargs)
         self._weight = None
         self._mask = None
@@ -605,7 +605,7 @@ class Target_Image(Image):
         if mask is not None:
             self.mask = mask
         if psf is not None:
-            self.psf = psf
+            self.set_psf(psf, psf_upscale)
 
     # variance / weight (target_image.py:117-184, 271-281)
     @property
@@ -656,11 +656,16 @@ class Target_Image(Image):
 
     @psf.setter
     def psf(self, psf):
-        """target_image.py:303-325."""
+        self.set_psf(psf)
+
+    def set_psf(self, psf, psf_upscale=1):
+        """target_image.py:303-325: a PSF given as data gets the pixelscale
+        of this image divided by ``psf_upscale``; models convolved with it are
+        sampled on a grid that much finer and summed back in blocks."""
         if psf is None or isinstance(psf, PSF_Image):
             self._psf = psf
         else:
-            self._psf = PSF_Image(data=psf, pixelscale=self.pixelscale)
+            self._psf = PSF_Image(data=psf, pixelscale=self.pixelscale / psf_upscale)
 
     def _new(self, data, window):
         return Target_Image(data=data, window=window, zeropoint=self.zeropoint, identity=self.identity)

@@ -192,6 +192,29 @@ int64_t apx_convolve_workspace_bytes(int32_t kh, int32_t kw);
 int apx_convolve(const apx_conv_job* job, const float* planes, const float* kernels,
                  void* workspace, int64_t workspace_bytes, void* stream);
 
+#define APX_MAX_BLOCK_COLS 16
+
+/* A model sampled on a PSF-upscaled grid (model_object.py:313-314, 348-349):
+ * Image.reduce, the sum of scale x scale blocks (image_object.py:348-376),
+ * then the window-aware += (image_object.py:450-457).  src is pixel-major,
+ * nc floats per fine pixel, src_w fine pixels per row.  For c < nc, y < h,
+ * x < w:
+ *   dst[(dst_y0 + y) * dst_row + (dst_x0 + x) * dst_pix + col[c]] +=
+ *     sum over a, b < scale of src[((scale * y + b) * src_w + scale * x + a) * nc + c] */
+typedef struct apx_block_sum {
+  int32_t scale, nc;       /* block side; channels per pixel (<= APX_MAX_BLOCK_COLS) */
+  int32_t w, h;            /* coarse box */
+  int32_t src_w, src_h;    /* src extent in fine pixels (>= scale * w, scale * h) */
+  float* dst;
+  int64_t dst_row;         /* dst row stride (floats), >= dst_w * dst_pix */
+  int32_t dst_pix;         /* dst pixel stride (floats) */
+  int32_t dst_w, dst_h;    /* dst extent (pixels); the box must fit inside */
+  int32_t dst_x0, dst_y0;  /* dst pixel of box pixel (0, 0) */
+  int32_t col[APX_MAX_BLOCK_COLS]; /* dst offset within a pixel of src channel c */
+} apx_block_sum;
+
+int apx_block_sum_add(const apx_block_sum* job, const float* src, void* stream);
+
 /* Workspace (bytes) for apx_chi2 / apx_normal_equations / apx_rpp. */
 int64_t apx_reduce_workspace_bytes(int64_t npix, int32_t np);
 

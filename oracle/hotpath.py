@@ -35,6 +35,9 @@ component_sample           flux conservation, whole-pixel translation, q=1
                            PA-independence, symmetry invariants
 simpsons sampling          exact on cubics (4-point Gauss-Legendre of the
                            same parallelogram pixel); centre sample = value
+PSF upscale                profile at the fine pixel centres, scipy
+                           convolve2d, explicit U x U block sums (the float32
+                           1/U grid of model_object.py:313-314 included)
 trapezoid sampling         closed form: a round Gaussian is separable, so the
                            corner mean is the product of 1-D endpoint means;
                            exact on bilinear functions of a sheared pixel;
@@ -255,6 +258,17 @@ class Grid:
         self.ref_ij = self.ref_ij - torch.tensor([bx, by], dtype=F64)
         return self
 
+    @property
+    def pixel_length(self):
+        """wcs.py:509-510: sqrt(|det P|)."""
+        return float(torch.sqrt(self.area))
+
+    def rescale(self, scale):
+        """window_object.py:233-239: pixelscale * scale, pixel_shape // scale
+        (floor of the exact quotient), reference pixel (ref + 0.5)/scale - 0.5."""
+        return Grid(int(self.nx // scale), int(self.ny // scale), self.P * scale,
+                    (self.ref_ij + 0.5) / scale - 0.5, self.ref_xy.clone())
+
     def crop_to_pixel(self, pixels):
         """window_object.py:358-365; pixels = [[xmin,xmax],[ymin,ymax]]."""
         p = torch.tensor(pixels, dtype=F64)
@@ -434,6 +448,7 @@ class Component:
     values: Dict[str, torch.Tensor]
     locked: Sequence[str] = ()
     psf: Optional[object] = None  # PSF image tensor, or a PSF-model Component
+    psf_upscale: int = 1  # PSF pixelscale = window pixelscale / psf_upscale (target_image.py:303-324)
     psf_mode: str = "none"
     psf_convolve_mode: str = "fft"
     sampling_mode: str = "midpoint"
@@ -617,8 +632,14 @@ def component_sample(comp, v, sample_window=None):
     center = v["center"]
     if comp.psf_mode == "full":
         psf = comp.psf
+        U = comp.psf_upscale
         if comp.aux is not None:
             psf, _ = psf_sample(comp.aux, v["psf"])
+            U = round(working.pixel_length / comp.aux.window.pixel_length)
+        if U != 1:
+            # model_object.py:313-314: 1 / psf_upscale of an int32 tensor is a
+            # float32 scalar, so the scale is float32(1/U), not 1/U
+            working = working.rescale(float(np.float32(1.0 / U)))
         bx, by = psf_border(psf)
         working.pad(bx, by)
         pc = working.plane_to_pixel(center)
@@ -635,6 +656,11 @@ def component_sample(comp, v, sample_window=None):
         working.ref_ij = working.ref_ij + shift
         data = data[by:data.shape[0] - by, bx:data.shape[1] - bx]
         working.crop(bx, by)
+        if U != 1:
+            # model_object.py:348-349 + image_object.py:348-376: sum U x U blocks
+            MS, NS = data.shape[0] // U, data.shape[1] // U
+            data = data[:MS * U, :NS * U].reshape(MS, U, NS, U).sum(dim=(1, 3))
+            working = working.rescale(U)
     else:
         reference, deep = sample_init(comp, working, v, center)
         data = sample_integrate(comp, deep, reference, working, v, center)

@@ -40,7 +40,8 @@ def test_struct_layouts_match_header():
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     from astrophot_amd import batch as B
     for cname, cls in (("apx_component", L.Component), ("apx_conv_job", L.ConvJob),
-                       ("apx_batch_galaxy", B.Galaxy), ("apx_batch", B.Batch), ("apx_group", L.Group)):
+                       ("apx_batch_galaxy", B.Galaxy), ("apx_batch", B.Batch), ("apx_group", L.Group),
+                       ("apx_block_sum", L.BlockSum)):
         body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), text, re.S).group(1)
         fields = []
         for decl in body.split(";"):
@@ -248,3 +249,31 @@ def test_unknown_sampling_mode_raises(monkeypatch):
                                               "Ie": 1.0})
     with pytest.raises(ap.errors.SpecificationConflict, match="trapezoid"):
         m()
+
+
+@pytest.mark.parametrize("U", [2, 3])
+def test_upscaled_render_geometry(monkeypatch, U):
+    """model_object.py:313-349: the working grid of a PSF U times finer than
+    the image is the window rescaled by float32(1/U) and padded by the PSF
+    border in fine pixels; the convolved fine box is summed back in U x U
+    blocks onto the image (one pixel short per axis for U = 3)."""
+    fake = _FakeLib()
+    monkeypatch.setattr(L, "load", lambda require_gpu=True: fake)
+    monkeypatch.setattr(L, "stream", lambda: 0)
+    psf = H.gaussian_psf(1.3, 13, 1.0 / U)
+    tgt = ap.image.Target_Image(data=np.zeros((26, 30)), pixelscale=0.8, psf=psf, psf_upscale=U)
+    assert float(tgt.psf.pixel_length) == pytest.approx(0.8 / U)
+    m = ap.models.AstroPhot_Model(name="s", model_type="sersic galaxy model", target=tgt, psf_mode="full",
+                                  parameters={"center": [12.3, 9.1], "q": 0.6, "PA": 0.7, "n": 2.5, "Re": 4.0,
+                                              "Ie": 1.0})
+    m()
+    s = float(np.float32(1.0 / U))
+    comp = [a for n, a in fake.calls if n == "apx_render_component"][0][0]._obj
+    nfx, nfy = int(30 // s), int(26 // s)
+    assert (comp.nx, comp.ny) == (nfx + 14, nfy + 14)  # 13 x 13 PSF: border 7 fine pixels
+    assert comp.P[0] == pytest.approx(0.8 * s, rel=1e-15) and comp.pixel_area == pytest.approx((0.8 * s) ** 2)
+    conv = [a for n, a in fake.calls if n == "apx_convolve"][0][0]._obj
+    bs = [a for n, a in fake.calls if n == "apx_block_sum_add"][0][0]._obj
+    assert (bs.scale, bs.nc, bs.w, bs.h) == (U, 1, nfx // U, nfy // U)
+    assert (conv.x0, conv.y0, conv.ow, conv.oh) == (7, 7, U * bs.w, U * bs.h)
+    assert (bs.dst_x0, bs.dst_y0, bs.dst_w, bs.dst_h) == (0, 0, 30, 26)

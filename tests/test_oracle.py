@@ -359,6 +359,46 @@ def test_trapezoid_sampling_exact_on_pixel_bilinear():
     assert torch.allclose(trapz, comp.evaluate(Cm[0], Cm[1], grid.P, None), rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("U", [2, 3])
+def test_psf_upscale_block_sums(U):
+    """model_object.py:313-349 with a PSF U times finer than the image
+    (target_image.py:303-324): the model is sampled on a grid of pixelscale
+    P * float32(1/U) and shape floor(n / float32(1/U)), convolved there, and
+    U x U blocks are summed back onto the image grid.  With U = 3 the float32
+    scale leaves the fine grid one pixel short (3n - 1), so the last row and
+    column of the image receive nothing.  Against a direct restatement:
+    profile values at the fine pixel centres, scipy convolve2d, explicit block
+    sums.  The centre sits on a fine pixel centre, so the sub-pixel shift is
+    zero and the shifted kernel is the PSF itself."""
+    from scipy.signal import convolve2d
+    nx, ny, sigma, flux = 12, 10, 2.0, 0.5
+    s = float(np.float32(1.0 / U))
+    nfx, nfy = int(nx // s), int(ny // s)
+    assert (nfx, nfy) == ((U * nx - 1, U * ny - 1) if U == 3 else (U * nx, U * ny))
+    rng = np.random.default_rng(U)
+    psf = rng.uniform(0.1, 1.0, size=(5, 5))
+    psf /= psf.sum()
+    kx, ky = 17, 13
+    c = [s * (kx + 0.5), s * (ky + 0.5)]
+    comp = H.Component("gaussian", H.Grid(nx, ny), {"center": c, "q": 1.0, "PA": 0.4, "sigma": sigma, "flux": flux},
+                       psf=torch.tensor(psf), psf_mode="full", psf_upscale=U, integrate_mode="none")
+    m = H.Model([comp])
+    got = m.forward(m.vector_representation()).numpy()
+    b = 3
+    xf = s * (np.arange(nfx + 2 * b) - b + 0.5) - c[0]
+    yf = s * (np.arange(nfy + 2 * b) - b + 0.5) - c[1]
+    r2 = xf[None, :] ** 2 + yf[:, None] ** 2 + 1e-6
+    f = s * s * 10 ** flux / math.sqrt(2 * math.pi * sigma ** 2) * np.exp(-r2 / (2 * sigma ** 2))
+    conv = convolve2d(f, psf, mode="same")[b:-b, b:-b]
+    want = np.zeros((ny, nx))
+    for j in range(nfy // U):
+        for i in range(nfx // U):
+            want[j, i] = conv[U * j:U * j + U, U * i:U * i + U].sum()
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-15 * want.max())
+    if U == 3:
+        assert np.all(got[-1, :] == 0) and np.all(got[:, -1] == 0)
+
+
 def _psf_comp(n=2.5, Rd=2.2, side=15, P=1.0, **kw):
     g = H.Grid(side, side, P, ref_ij=((side - 1) / 2, (side - 1) / 2), ref_xy=(0.0, 0.0))
     kw.setdefault("sampling_mode", "simpsons")

@@ -296,10 +296,44 @@ class Component_Model(AstroPhot_Model):
             raise NotImplementedError("PSF convolution in sub-window not available yet")
         if self.psf_convolve_mode not in ("fft", "direct"):
             raise ValueError(f"unrecognized psf_convolve_mode: {self.psf_convolve_mode}")
-        _engine.render(self, self._values(), _engine.RenderRequest(window=working_window, image=image))
-        if self.mask is not None:
-            image.data *= torch.logical_not(torch.as_tensor(self.mask, device=image.data.device))
+        if self.mask is None:
+            _engine.render(self, self._values(), _engine.RenderRequest(window=working_window, image=image))
+            return image
+        # model_object.py:370-373: the mask zeroes this model's own samples
+        # before they are added to the image
+        part = Model_Image(window=working_window.copy())
+        _engine.render(self, self._values(), _engine.RenderRequest(window=working_window, image=part))
+        part.data *= self._keep(working_window)
+        image += part
         return image
+
+    def _keep(self, window):
+        """1 where the model mask lets a sample of ``window`` through, else 0
+        (float32, device); the mask covers the sampled window."""
+        m = torch.as_tensor(self.mask, device=AP_config.ap_device)
+        if tuple(m.shape) != (window.ny, window.nx):
+            raise SpecificationConflict(
+                f"{self.name}: mask shape {tuple(m.shape)} does not match the sampled window {(window.ny, window.nx)}")
+        return torch.logical_not(m).to(torch.float32)
+
+    def _mask_columns(self, jac, window, cols):
+        """Zero this model's Jacobian columns ``cols`` at its masked pixels:
+        the forward-mode derivative of the masked sample."""
+        keep = self._keep(window)
+        (dy, dx), (sy, sx), (h, w) = _engine._box(jac.window, window, window)
+        for c in sorted(set(cols)):
+            jac.data[dy:dy + h, dx:dx + w, c] *= keep[sy:sy + h, sx:sx + w]
+
+    def _masked_G(self, G, G_window, window):
+        """G with this model's masked pixels of ``window`` set to zero: the
+        adjoint of the masked sample."""
+        if self.mask is None:
+            return G
+        keep = self._keep(window)
+        (dy, dx), (sy, sx), (h, w) = _engine._box(G_window, window, window)
+        G = G.clone()
+        G[dy:dy + h, dx:dx + w] *= keep[sy:sy + h, sx:sx + w]
+        return G
 
     def jacobian(self, parameters: Optional[torch.Tensor] = None, as_representation: bool = False,
                  window: Optional[Window] = None, pass_jacobian: Optional[Jacobian_Image] = None, **kw):
@@ -325,6 +359,8 @@ class Component_Model(AstroPhot_Model):
         slot_cols = {s: jac.parameters.index(pid) for (s, _), pid in zip(slots, ids)}
         dvdr = {s: (d if as_representation else 1.0) for s, d in slots}
         self._jacobian_windows(window, jac, slot_cols, dvdr)
+        if self.mask is not None:
+            self._mask_columns(jac, window, slot_cols.values())
         return jac
 
     def _jacobian_windows(self, window, jac, slot_cols, dvdr):
@@ -365,6 +401,7 @@ class Component_Model(AstroPhot_Model):
         slot_cols = {s: k for k, (s, _) in enumerate(slots)}
         dvdr = {s: (d if as_representation else 1.0) for s, d in slots}
         sample_window = self.window & G_window
+        G = self._masked_G(G, G_window, sample_window)
         out, col_index = _engine.vjp(self, self._values(), sample_window, G, G_window, slot_cols, dvdr)
         return out[[col_index[k] for k in range(n)]]
 
@@ -782,7 +819,8 @@ class Group_Model(AstroPhot_Model):
             slots = model._free_slots()
             slot_cols = {sl: k for k, (sl, _) in enumerate(slots)}
             dvdr = {sl: (d if as_representation else 1.0) for sl, d in slots}
-            out, col_index = _engine.vjp(model, model._values(), G_window, G, G_window, slot_cols, dvdr,
+            Gm = model._masked_G(G, G_window, G_window) if isinstance(model, Component_Model) else G
+            out, col_index = _engine.vjp(model, model._values(), G_window, Gm, G_window, slot_cols, dvdr,
                                          region=model.window & G_window)
             parts.append(out[[col_index[k] for k in range(n)]])
         return torch.cat(parts) if parts else torch.zeros(0, dtype=torch.float64, device=AP_config.ap_device)

@@ -336,3 +336,59 @@ def test_autograd_function_group():
         b[k] -= h
         fd = ((((om.forward(a) - 1.0) ** 2 * W).sum() - ((om.forward(b) - 1.0) ** 2 * W).sum()) / (2 * h)).item()
         assert abs(r2.grad[k].item() - fd) <= 2e-3 * max(abs(fd), ref.abs().max().item() * 1e-3), (k, fd)
+
+
+def _masked_pair():
+    nx, ny = 40, 36
+    mask = torch.zeros((ny, nx))
+    mask[10, 13] = 1
+    mask[17:20, 19:23] = 1
+    tgt = target(nx, ny, psf=PSF11)
+    p = dict(GAL["sersic"])
+    plain = ap.models.AstroPhot_Model(name="plain", model_type="sersic galaxy model", target=tgt, parameters=dict(p),
+                                      psf_mode="full")
+    masked = ap.models.AstroPhot_Model(name="masked", model_type="sersic galaxy model", target=tgt,
+                                       parameters=dict(p), psf_mode="full", mask=mask)
+    return tgt, mask, plain, masked
+
+
+def test_model_mask():
+    """model_object.py:370-371 (reference test_model.py:139-161): masked
+    samples are zero, the others are the unmasked model's; the Jacobian is the
+    derivative of the masked sample and the adjoint its transpose."""
+    tgt, mask, plain, masked = _masked_pair()
+    keep = torch.logical_not(mask).float().cuda()
+    img = masked().data
+    assert img[10, 13].item() == 0.0 and img[11, 12].item() != 0.0
+    assert torch.equal(img, plain().data * keep)
+    J = masked.jacobian().data
+    Jp = plain.jacobian().data
+    assert torch.equal(J, Jp * keep[..., None])
+    G = torch.tensor(np.random.default_rng(4).normal(size=(36, 40)), dtype=torch.float32)
+    rep = masked.parameters.vector_representation().clone()
+    got = masked.vjp(G, parameters=rep.clone(), as_representation=True).cpu()
+    Jr = masked.jacobian(parameters=rep.clone(), as_representation=True).data.double().cpu()
+    jtg = (Jr * G.double()[..., None]).sum(dim=(0, 1))
+    assert ((got - jtg).abs() / jtg.abs().max()).max() < JAC_TOL
+
+
+def test_masked_member_keeps_other_light():
+    """A group sampled into an image it is handed calls each member on the
+    shared working image (group_model_object.py:217-229): a member's mask
+    zeroes only its own samples (model_object.py:370-373), not the light the
+    other members put there."""
+    tgt, mask, plain, masked = _masked_pair()
+    sky = ap.models.AstroPhot_Model(name="sky", model_type="flat sky model", target=tgt,
+                                    parameters={"F": math.log10(0.5)})
+    grp = ap.models.AstroPhot_Model(name="g", model_type="group model", models=[sky, masked], target=tgt)
+    img = grp.make_model_image()
+    grp.sample(image=img)
+    keep = torch.logical_not(mask).float().cuda()
+    want = sky().data + plain().data * keep
+    assert float((img.data - want).abs().max()) <= 1e-6 * float(want.abs().max())
+    assert img.data[10, 13].item() == pytest.approx(sky().data[10, 13].item(), rel=1e-6)
+    own = grp()
+    assert float((own.data - want).abs().max()) <= 1e-6 * float(want.abs().max())
+    # group Jacobian: the masked member's columns vanish at its masked pixels
+    J = grp.jacobian().data
+    assert float(J[10, 13, 1:].abs().max()) == 0.0 and J[10, 13, 0].item() != 0.0

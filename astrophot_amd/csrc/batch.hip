@@ -154,9 +154,14 @@ __global__ __launch_bounds__(256) void build_kernel(apx_batch b, Scratch s, int 
   if (g >= b.ngal) return;
   s.gal_cnt[g] = 0;
   s.gal_rcnt[g] = 0;
-  const int ph = st_of(b, g)->phase;
+  const GalState* st = st_of(b, g);
+  const int ph = st->phase;
+  // the probe point's rpp only enters the step through the acceleration
+  // term, which lm.py:283-286 zeroes for L <= 1e-4: such fits skip the probe
   const bool active = stage == ST_ALL || stage == ST_ALL_JAC ||
-                      (stage == ST_JAC ? (ph == PH_INIT || ph == PH_NEEDJ) : ph == PH_SUB);
+                      (stage == ST_JAC     ? (ph == PH_INIT || ph == PH_NEEDJ)
+                       : stage == ST_PROBE ? (ph == PH_SUB && st->L > 1e-4)
+                                           : ph == PH_SUB);
   if (!active) return;
   const apx_batch_galaxy G = b.gal[g];
   s.gal_list[atomicAdd(&s.cnt[0], 1)] = g;

@@ -33,13 +33,42 @@ STATUS = {0: "running", 1: "success", 2: "success by immobility. Convergence not
           3: "fail. Could not find step to improve Chi^2", 4: "fail. Maximum iterations", 5: "skipped"}
 
 
+CONV_IMPL = {0: "default", 1: "valu", 2: "mfma", 3: "sep"}
+
+
+def rank_one_factors(psfs, rtol: float = 1e-6):
+    """(u, v) with psf = outer(u, v) to ``rtol`` of the PSF's largest value,
+    or None, for each PSF: the skeleton through the largest element,
+    u = P[:, c*], v = P[r*, :] / P[r*, c*], which reproduces a rank-one image
+    exactly and is checked against the image itself."""
+    out = [None] * len(psfs)
+    groups: Dict[tuple, List[int]] = {}
+    for k, p in enumerate(psfs):
+        groups.setdefault(tuple(np.shape(p)), []).append(k)
+    for (h, w), idx in groups.items():
+        P = np.stack([np.asarray(psfs[k], dtype=np.float64) for k in idx])      # (m, h, w)
+        flat = np.abs(P).reshape(len(idx), -1).argmax(axis=1)
+        r, c = flat // w, flat % w
+        m = np.arange(len(idx))
+        piv = P[m, r, c]
+        safe = np.where(piv == 0, 1.0, piv)
+        u = P[m, :, c]                                                          # (m, h)
+        v = P[m, r, :] / safe[:, None]                                          # (m, w)
+        err = np.abs(u[:, :, None] * v[:, None, :] - P).reshape(len(idx), -1).max(axis=1)
+        ok = (piv != 0) & (err <= rtol * np.abs(piv))
+        for j, k in enumerate(idx):
+            if ok[j]:
+                out[k] = (u[j], v[j])
+    return out
+
+
 class Galaxy(ctypes.Structure):
     """apx_batch_galaxy."""
 
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ph", ctypes.c_int32), ("pw", ctypes.c_int32),
                 ("pix_off", ctypes.c_int64), ("psf_off", ctypes.c_int64), ("work_off", ctypes.c_int64),
                 ("tap_off", ctypes.c_int64), ("chunk_off", ctypes.c_int64), ("tile_off", ctypes.c_int64),
-                ("origin", ctypes.c_double * 2), ("ref_npix", ctypes.c_int64)]
+                ("origin", ctypes.c_double * 2), ("ref_npix", ctypes.c_int64), ("sep_off", ctypes.c_int64)]
 
 
 class Batch(ctypes.Structure):
@@ -52,10 +81,11 @@ class Batch(ctypes.Structure):
                 ("P", ctypes.c_double * 4), ("softening", ctypes.c_double), ("tolerance", ctypes.c_double),
                 ("quad_level", ctypes.c_int32), ("gridding", ctypes.c_int32), ("max_depth", ctypes.c_int32),
                 ("max_iter", ctypes.c_int32), ("relative_tolerance", ctypes.c_double), ("L0", ctypes.c_double),
-                ("gal", _p), ("data", _p), ("weight", _p), ("psf", _p), ("model0", _p), ("model1", _p),
+                ("gal", _p), ("data", _p), ("weight", _p), ("psf", _p), ("psf_sep", _p), ("model0", _p),
+                ("model1", _p),
                 ("jac", _p), ("planes", _p), ("taps", _p), ("state", _p), ("scratch", _p),
                 ("conv_impl", ctypes.c_int32), ("conv_blocks", ctypes.c_int32), ("quad_blocks", ctypes.c_int32),
-                ("quad_group", ctypes.c_int32), ("chunk_blocks", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("quad_group", ctypes.c_int32), ("chunk_blocks", ctypes.c_int32), ("sep_all", ctypes.c_int32)]
 
 
 SIGNATURES = {
@@ -138,6 +168,17 @@ class BatchFit:
         b.softening, b.tolerance = softening, sampling_tolerance
         b.quad_level, b.gridding, b.max_depth = integrate_quad_level, integrate_gridding, integrate_max_depth
         b.max_iter, b.relative_tolerance, b.L0 = max_iter, relative_tolerance, L0
+        # rank-one factors of every PSF that has them (the separable convolution)
+        factors = rank_one_factors([c.psf for c in cutouts])
+        self.sep_all = all(f is not None for f in factors)
+        if self.sep_all:
+            off = 0
+            for k, (u, v) in enumerate(factors):
+                gal[k].sep_off = off
+                off += len(u) + len(v)
+            self.psf_sep = torch.as_tensor(np.concatenate([np.concatenate(f) for f in factors]),
+                                           dtype=torch.float32).to(dev)
+            b.psf_sep, b.sep_all = self.psf_sep.data_ptr(), 1
         self.gal_host = gal
         f32 = dict(dtype=torch.float32, device=dev)
         self.gal_dev = torch.frombuffer(bytearray(gal), dtype=torch.uint8).to(dev)
@@ -152,6 +193,7 @@ class BatchFit:
             self.weight = torch.cat([(1.0 / torch.as_tensor(c.variance, dtype=torch.float64)).to(torch.float32)
                                      .reshape(-1) for c in cutouts]).to(dev)
         self.psf = torch.cat([torch.as_tensor(c.psf, dtype=torch.float32).reshape(-1) for c in cutouts]).to(dev)
+
         self.model0 = torch.zeros(b.npix, **f32)
         self.model1 = torch.zeros(b.npix, **f32)
         self.jac = torch.zeros(b.npix * NP, **f32)
@@ -241,14 +283,14 @@ class BatchFit:
         b = self.b
         log = {"conv": {}, "quad": {}, "chunk": {}}
         self.jacobian(rep, out=out, parts=1)
-        impls = [1] + ([2] if b.max_kernel <= 35 else [])
+        impls = [1] + ([2] if b.max_kernel <= 35 else []) + ([3] if self.sep_all and b.max_kernel <= 61 else [])
         best = None
         conv_grid = sorted({min(c, int(b.ntile)) for c in (1024, 2048, 4096, 8192, 16384)})  # launches clamp to ntile
         for impl in impls:
             for blocks in conv_grid:
                 b.conv_impl, b.conv_blocks = impl, blocks
                 ms = self._time(lambda: self.jacobian(rep, out=out, parts=2))
-                log["conv"][f"{'mfma' if impl == 2 else 'valu'}:{blocks}"] = ms
+                log["conv"][f"{CONV_IMPL[impl]}:{blocks}"] = ms
                 if best is None or ms < best[0]:
                     best = (ms, impl, blocks)
         b.conv_impl, b.conv_blocks = best[1], best[2]
@@ -271,7 +313,7 @@ class BatchFit:
                 best = (ms, blocks)
         b.chunk_blocks = best[1]
         self.graph = None  # a captured tick holds the old launch shape
-        choice = {"conv_impl": "mfma" if b.conv_impl == 2 else "valu", "conv_blocks": b.conv_blocks,
+        choice = {"conv_impl": CONV_IMPL[b.conv_impl], "conv_blocks": b.conv_blocks,
                   "quad_group": b.quad_group or (32 if b.gridding * b.gridding <= 32 else 64),
                   "quad_blocks": b.quad_blocks, "chunk_blocks": b.chunk_blocks}
         self.tuning = {"choice": choice, "timings_ms": log}

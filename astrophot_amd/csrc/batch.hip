@@ -230,6 +230,54 @@ __global__ __launch_bounds__(256) void eval_kernel(apx_batch b, Scratch s, const
   s.ev[g] = e;
 }
 
+// 1-D linear interpolation of the zero-padded factor f (n values, padded to
+// n + 2) at r - s, with the clamped corners of bilinear(); d = d value / d s
+__device__ __forceinline__ void lin1d(const float* f, int n, int r, double s, double& val, double& d) {
+  const double x = (double)r - s;
+  const int xf = (int)floor(x);
+  const int x0 = min(max(xf, 0), n), x1 = min(max(xf + 1, 1), n + 1);
+  const double f0 = (x0 >= 1 && x0 <= n) ? (double)f[x0 - 1] : 0.0;
+  const double f1 = (x1 >= 1 && x1 <= n) ? (double)f[x1 - 1] : 0.0;
+  val = f0 * ((double)x1 - x) + f1 * (x - (double)x0);
+  d = -(f1 - f0);
+}
+
+// Separable kernels of a rank-one PSF P = u v^T.  bilinear() interpolates the
+// padded PSF with corners clamped per axis, so the shifted PSF is U V^T with U
+// (V) the 1-D interpolation of u (v) at the row (column) shift, and with
+// K = U V^T / (sum U sum V) (_model_methods.py:187-207, 243):
+//   K = uh vh^T,  dK/ds_x = uh vh'^T,  dK/ds_y = uh' vh^T,
+//   uh = U / sum U,  uh' = dU / sum U - U sum dU / (sum U)^2  (likewise v).
+// Wave 0 writes uh and uh', wave 1 vh and vh', flipped for correlation, at
+// taps + tap_off as [uh (kh) | vh (kw) | uh' (kh) | vh' (kw)].
+__device__ void prepare_separable(const apx_batch& b, const apx_batch_galaxy& G, double sx, double sy) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (w >= 2) return;
+  const int H2 = G.ph + 2, W2 = G.pw + 2;
+  const int n = w == 0 ? G.ph : G.pw, N2 = n + 2;
+  const float* f = b.psf_sep + G.sep_off + (w == 0 ? 0 : G.ph);
+  const double sh = w == 0 ? sy : sx;
+  double S = 0.0, Sd = 0.0;
+  for (int r = lane; r < N2; r += 64) {
+    double v, d;
+    lin1d(f, n, r, sh, v, d);
+    S += v;
+    Sd += d;
+  }
+  S = wsum(S);
+  Sd = wsum(Sd);
+  const double iS = 1.0 / S;
+  float* kt = b.taps + G.tap_off;
+  float* o = kt + (w == 0 ? 0 : H2);
+  float* od = kt + H2 + W2 + (w == 0 ? 0 : H2);
+  for (int r = lane; r < N2; r += 64) {
+    double v, d;
+    lin1d(f, n, r, sh, v, d);
+    o[N2 - 1 - r] = (float)(v * iS);
+    od[N2 - 1 - r] = (float)(d * iS - v * Sd * iS * iS);
+  }
+}
+
 // per fit (one workgroup each): shifted PSF taps in the conv layout
 __global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, int stage) {
   __shared__ double red[4];
@@ -240,6 +288,10 @@ __global__ __launch_bounds__(256) void prepare_kernel(apx_batch b, Scratch s, in
     const int g = s.gal_list[k];
     const apx_batch_galaxy G = b.gal[g];
     const double sx = s.ev[g].sx, sy = s.ev[g].sy;
+    if (b.conv_impl == 3) {
+      prepare_separable(b, G, sx, sy);
+      continue;
+    }
     // shifted, normalised PSF (and its centre derivatives), flipped, in the
     // KT[k][b][u] = Kf[u - 7][b] layout of conv_common.h
     const int H2 = G.ph + 2, W2 = G.pw + 2, n = H2 * W2, kts = fast_kts(H2);
@@ -1085,6 +1137,166 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
   }
 }
 
+// ---------------------------------------------------------------------------
+// Separable convolution (conv_impl 3, every PSF rank one; prepare_separable):
+// per plane of the 64 x kTR tile, a column pass
+//   hu[t][u] = sum_a uh_f[a] in[t + a][u]      (u over the 64 + kw - 1 staged columns)
+// and a row pass  out[t][x] = sum_b vh_f[b] hu[t][x + b],  2 (kh + kw) flops an
+// output instead of 2 kh kw.  In the Jacobian stage the value plane also takes
+// the column pass with uh' (hu2) and gives, with C_x = hu * vh' and
+// C_y = hu2 * vh, the centre columns P^-1-mixed as the MFMA taps are:
+//   dm/dcx = P^-1[0][0] C_x + P^-1[1][0] C_y,  dm/dcy = P^-1[0][1] C_x + P^-1[1][1] C_y.
+// Thread t owns outputs (row t / 16, columns t % 16 + 16 i); the row pitch
+// of hu is 16 mod 64 floats so a wave's 4 rows x 16 columns meet distinct banks.
+constexpr int kSepMax = 61;  // widest kernel the separable convolution takes (LDS)
+__host__ __device__ inline int sp_pitch(int cols) { return 16 + 64 * ((cols - 16 + 63) / 64); }
+__host__ inline size_t sep_lds_bytes(int kmax, bool jac) {
+  const int rows = kTR + kmax - 1, cols = 64 + kmax - 1;
+  return sizeof(float) * ((size_t)rows * cols + (jac ? 2 : 1) * (size_t)kTR * sp_pitch(cols) + 4 * (size_t)kmax);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void sconv_kernel(apx_batch b, Scratch s, float* out) {
+  extern __shared__ float smem[];
+  __shared__ double redd[4][8];  // kConvStaticLds
+  constexpr bool JAC = MODE == EPI_NORMAL;
+  constexpr int nchan = JAC ? kConvChan : 1;
+  constexpr int W = MODE == EPI_RPP ? NP : 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  const double det = b.P[0] * b.P[3] - b.P[1] * b.P[2];
+  const float pi00 = (float)(b.P[3] / det), pi01 = (float)(-b.P[1] / det);
+  const float pi10 = (float)(-b.P[2] / det), pi11 = (float)(b.P[0] / det);
+  for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
+    const int2 e = s.tile_list[k];
+    const int g = __builtin_amdgcn_readfirstlane(e.x);
+    const apx_batch_galaxy G = b.gal[g];
+    const GalState* st = st_of(b, g);
+    const float ie_scale = JAC ? (float)(kLn10 * s.ev[g].dvdr[NP - 1]) : 0.0f;
+    const int nxw = G.nx + G.pw + 1, nyw = G.ny + G.ph + 1;  // cutout + 2 * border
+    const int ntx = (G.nx + 63) / 64;
+    const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kTR;
+    const int kh = G.ph + 2, kw = G.pw + 2;
+    const int rows = kTR + kh - 1, cols = 64 + kw - 1, hp = sp_pitch(cols);
+    const int64_t tile = G.tile_off + e.y;
+    float* tin = smem;                          // rows x cols
+    float* hu = tin + rows * cols;              // kTR x hp
+    float* hu2 = hu + kTR * hp;                 // kTR x hp (Jacobian stage, value plane)
+    float* fac = hu + (JAC ? 2 : 1) * kTR * hp; // uh | vh | uh' | vh'
+    const float* fu = fac, *fv = fac + kh, *fdu = fac + kh + kw, *fdv = fac + 2 * kh + kw;
+    __syncthreads();  // the previous tile's readers are done with LDS
+    for (int q = threadIdx.x; q < 2 * (kh + kw); q += 256) fac[q] = b.taps[G.tap_off + q];
+    const int oy = oy0 + ty;
+    bool valid[4];
+    int64_t pp[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ox = ox0 + tx + 16 * i;
+      valid[i] = ox < G.nx && oy < G.ny;
+      pp[i] = G.pix_off + (int64_t)oy * G.nx + ox;
+    }
+    double rd[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) rd[i] = 0.0;
+    for (int c = 0; c < nchan; ++c) {
+      const float* plane = b.planes + (int64_t)c * b.nwork + G.work_off;
+      const bool two = JAC && c == 0;
+      __syncthreads();
+      // staging: 8 independent loads in flight per thread, then the LDS stores
+      for (int q0 = threadIdx.x; q0 < rows * cols; q0 += 8 * 256) {
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int q = q0 + 256 * i;
+          const int t = q / cols, u = q - t * cols;
+          const int gy = oy0 + t, gx = ox0 + u;
+          v[i] = (q < rows * cols && gy < nyw && gx < nxw) ? plane[(int64_t)gy * nxw + gx] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (q0 + 256 * i < rows * cols) tin[q0 + 256 * i] = v[i];
+      }
+      __syncthreads();
+      // column pass, 4 rows of one staged column a thread
+      for (int q = threadIdx.x; q < 4 * cols; q += 256) {
+        const int u = q % cols, t0 = 4 * (q / cols);
+        const float* col = tin + t0 * cols + u;
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, d0 = 0.0f, d1 = 0.0f, d2 = 0.0f, d3 = 0.0f;
+        float r0 = col[0], r1 = col[cols], r2 = col[2 * cols];
+#pragma unroll 4
+        for (int a = 0; a < kh; ++a) {
+          const float r3 = col[(a + 3) * cols];
+          const float f = fu[a];
+          a0 = fmaf(f, r0, a0);
+          a1 = fmaf(f, r1, a1);
+          a2 = fmaf(f, r2, a2);
+          a3 = fmaf(f, r3, a3);
+          if (two) {
+            const float fd = fdu[a];
+            d0 = fmaf(fd, r0, d0);
+            d1 = fmaf(fd, r1, d1);
+            d2 = fmaf(fd, r2, d2);
+            d3 = fmaf(fd, r3, d3);
+          }
+          r0 = r1;
+          r1 = r2;
+          r2 = r3;
+        }
+        float* h = hu + t0 * hp + u;
+        h[0] = a0;
+        h[hp] = a1;
+        h[2 * hp] = a2;
+        h[3 * hp] = a3;
+        if (two) {
+          float* h2 = hu2 + t0 * hp + u;
+          h2[0] = d0;
+          h2[hp] = d1;
+          h2[2 * hp] = d2;
+          h2[3 * hp] = d3;
+        }
+      }
+      __syncthreads();
+      // row pass
+      float o[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ox_[4] = {0.0f, 0.0f, 0.0f, 0.0f}, oy_[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      const float* h = hu + ty * hp + tx;
+      const float* h2 = hu2 + ty * hp + tx;
+#pragma unroll 4
+      for (int bb = 0; bb < kw; ++bb) {
+        const float f = fv[bb];
+        const float fd = two ? fdv[bb] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float v = h[16 * i + bb];
+          o[i] = fmaf(f, v, o[i]);
+          if (two) {
+            ox_[i] = fmaf(fd, v, ox_[i]);
+            oy_[i] = fmaf(f, h2[16 * i + bb], oy_[i]);
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (!valid[i]) continue;
+        if (JAC) {
+          if (c == 0) {
+            out[pp[i]] = o[i];
+            b.jac[(int64_t)(NP - 1) * b.npix + pp[i]] = ie_scale * o[i];
+            b.jac[pp[i]] = pi00 * ox_[i] + pi10 * oy_[i];
+            b.jac[b.npix + pp[i]] = pi01 * ox_[i] + pi11 * oy_[i];
+          } else {
+            b.jac[(int64_t)(c + 1) * b.npix + pp[i]] = o[i];
+          }
+        } else if (MODE == EPI_OUT) {
+          out[pp[i]] = o[i];
+        } else {
+          epilogue_pixel<MODE>(b, st, pp[i], o[i], rd);
+        }
+      }
+    }
+    if (MODE == EPI_RPP || MODE == EPI_CHI2) tile_partial<W>(redd, rd, s.part + tile * kPart, lane, w);
+  }
+}
+
 // per tile: J^T W J (MFMA 16x16x4 f32, 4 pixels per step, 16 kTR pixels per
 // wave), J^T W (Y - m0) and chi2(m0), into the tile's partial sums
 __global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
@@ -1566,13 +1778,17 @@ void render_part(const apx_batch& b, const Scratch& s, int stage, const double* 
 void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipStream_t stream,
                bool reduce) {
   // MFMA convolution unless the caller asks for the VALU one or the PSF is too wide
-  const bool mf = b.conv_impl != 1 && b.max_kernel <= kMfmaMax;
-  const size_t lds = mf ? mfma_lds_bytes(b.max_kernel, mode == EPI_NORMAL ? 3 : 1) : conv_lds_bytes(b.max_kernel);
+  const bool sep = b.conv_impl == 3;
+  const bool mf = !sep && b.conv_impl != 1 && b.max_kernel <= kMfmaMax;
+  const size_t lds = sep  ? sep_lds_bytes(b.max_kernel, mode == EPI_NORMAL)
+                     : mf ? mfma_lds_bytes(b.max_kernel, mode == EPI_NORMAL ? 3 : 1)
+                          : conv_lds_bytes(b.max_kernel);
   const int tc = (int)std::min<int64_t>(b.ntile, b.conv_blocks > 0 ? b.conv_blocks : 2 * kPersist);
-#define APX_CONV(M)                                                                              \
-  do {                                                                                           \
-    if (mf) hipLaunchKernelGGL(conv_mfma_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out); \
-    else hipLaunchKernelGGL(conv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);        \
+#define APX_CONV(M)                                                                                \
+  do {                                                                                             \
+    if (sep) hipLaunchKernelGGL(sconv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);     \
+    else if (mf) hipLaunchKernelGGL(conv_mfma_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out); \
+    else hipLaunchKernelGGL(conv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);          \
   } while (0)
   switch (mode) {
     case EPI_OUT:
@@ -1658,8 +1874,13 @@ static int check_batch(const apx_batch* b) {
       b->gridding > APX_MAX_GRIDDING || b->max_depth < 1 || b->max_depth > APX_MAX_DEPTH)
     return set_error(APX_E_LIMIT, "batch: integration settings out of range");
   if (b->max_kernel > kFastMax) return set_error(APX_E_LIMIT, "batch: psf too large");
-  if (b->conv_impl < 0 || b->conv_impl > 2 || (b->conv_impl == 2 && b->max_kernel > kMfmaMax))
-    return set_error(APX_E_INVALID, "batch: conv_impl must be 0, 1 or 2 (2 needs psf <= 33 px)");
+  if (b->conv_impl < 0 || b->conv_impl > 3 || (b->conv_impl == 2 && b->max_kernel > kMfmaMax))
+    return set_error(APX_E_INVALID, "batch: conv_impl must be 0, 1, 2 or 3 (2 needs psf <= 33 px)");
+  if (b->conv_impl == 3 &&
+      (!b->sep_all || !b->psf_sep || b->max_kernel > kSepMax ||
+       sep_lds_bytes(b->max_kernel, true) + kConvStaticLds > 65536))
+    return set_error(APX_E_INVALID,
+                     "batch: conv_impl 3 needs rank-one factors of every PSF (sep_all, psf_sep) and psf <= 59 px");
   if (b->quad_group != 0 && b->quad_group != 32 && b->quad_group != 64)
     return set_error(APX_E_INVALID, "batch: quad_group must be 0, 32 or 64");
   if (b->quad_group == 32 && b->gridding * b->gridding > 32)

@@ -272,6 +272,9 @@ typedef struct apx_batch_galaxy {
                          threshold (sersic_model.py:87-89); 0 = this cutout's working
                          grid.  A source sampled as part of a larger window (a group,
                          group_model_object.py:183-231) sets that window's working grid */
+  int64_t sep_off;    /* first float of this PSF's rank-one factors in psf_sep (ph column
+                         values u, then pw row values v, PSF = u v^T to float32
+                         rounding); read only by the separable convolution */
 } apx_batch_galaxy;
 
 typedef struct apx_batch {
@@ -289,6 +292,7 @@ typedef struct apx_batch {
   const float* data;           /* npix */
   const float* weight;         /* npix, 1/variance */
   const float* psf;            /* sum of ph * pw */
+  const float* psf_sep;        /* rank-one PSF factors (sep_off); NULL unless sep_all */
   float* model0;               /* npix: model at the current state */
   float* model1;               /* npix: spare model buffer (the LM ticks reduce their
                                   probe and trial renders in place, unstored) */
@@ -299,12 +303,13 @@ typedef struct apx_batch {
   void* state;                 /* ngal x apx_batch_state_bytes() */
   void* scratch;               /* apx_batch_scratch_bytes(batch) */
   /* launch shape (0 = built-in default); set by the caller's autotuner */
-  int32_t conv_impl;           /* 1 = VALU convolution, 2 = MFMA (kernels <= 35 px) */
+  int32_t conv_impl;           /* 1 = VALU convolution, 2 = MFMA (kernels <= 35 px),
+                                  3 = separable (sep_all, kernels <= 61 px) */
   int32_t conv_blocks;         /* workgroups of the convolution launches */
   int32_t quad_blocks;         /* workgroups of the level-2/3 quadrature launches */
   int32_t quad_group;          /* lanes per refined pixel: 32 (gridding <= 5) or 64 */
   int32_t chunk_blocks;        /* workgroups of the per-chunk render launches */
-  int32_t reserved;
+  int32_t sep_all;             /* 1: every PSF of the batch has factors in psf_sep */
 } apx_batch;
 
 /* Fill the offsets of gal[] (host array) and the totals of *batch. */

@@ -120,10 +120,14 @@ def _problem(n, seed):
     return cuts, np.stack(starts)
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_batch_fits_match_single_model_lm(graph):
+@pytest.mark.parametrize("graph,impl", [(False, 0), (True, 0), (True, 3)])
+def test_batch_fits_match_single_model_lm(graph, impl):
+    """impl 0: the default (MFMA) convolution; impl 3: the separable one,
+    whose probe and trial epilogues run inside the captured LM ticks."""
     cuts, starts = _problem(6, seed=2)
     bf = BatchFit(cuts)
+    assert bf.sep_all
+    bf.b.conv_impl = impl
     res = bf.fit(values_to_rep(starts), use_graph=graph)
     assert (res["status"] > 0).all(), res["status"]
     degenerate = []
@@ -208,7 +212,8 @@ def test_launch_shapes_agree_and_autotune():
     bf = BatchFit(cuts)
     rep = torch.tensor(values_to_rep(np.stack(truths)), dtype=torch.float64, device="cuda")
     ref_m, ref_j = [t.clone() for t in bf.jacobian(rep)]
-    for impl, grp, cb, qb, kb in ((1, 0, 0, 0, 0), (2, 64, 7, 3, 5), (1, 64, 1, 1, 1), (2, 0, 4096, 4096, 9)):
+    for impl, grp, cb, qb, kb in ((1, 0, 0, 0, 0), (2, 64, 7, 3, 5), (1, 64, 1, 1, 1), (2, 0, 4096, 4096, 9),
+                                  (3, 0, 0, 0, 0), (3, 64, 5, 2, 3)):
         b = bf.b
         b.conv_impl, b.quad_group, b.conv_blocks, b.quad_blocks, b.chunk_blocks = impl, grp, cb, qb, kb
         m, j = bf.jacobian(rep)
@@ -217,8 +222,34 @@ def test_launch_shapes_agree_and_autotune():
         assert float(((j - ref_j).abs().amax(dim=0) / scale).max()) < 1e-6, (impl, grp)
     tuning = bf.autotune(rep)
     ch = tuning["choice"]
-    assert ch["conv_impl"] in ("mfma", "valu") and ch["quad_group"] in (32, 64)
+    assert ch["conv_impl"] in ("mfma", "valu", "sep") and ch["quad_group"] in (32, 64)
     assert ch["conv_blocks"] > 0 and ch["quad_blocks"] > 0 and ch["chunk_blocks"] > 0
     m, j = bf.jacobian(rep)
     assert float((m - ref_m).abs().max() / ref_m.abs().max()) < 1e-6
 
+
+
+def test_separable_convolution_sheared_pixels_and_refusal():
+    """conv_impl 3 on a sheared pixelscale (the centre columns mix the two
+    separable derivative convolutions through P^-1, as the MFMA taps do)
+    agrees with the MFMA convolution to float32 summation order; a batch with
+    a PSF that is not rank one refuses conv_impl 3."""
+    truths, shapes, rng = make_cutouts(4, seed=13)
+    P = [[0.9, 0.12], [-0.07, 1.1]]
+    cuts = [Cutout(np.zeros((s, s)), np.ones((s, s)), psf) for s, psf in shapes]
+    bf = BatchFit(cuts, pixelscale=P)
+    rep = torch.tensor(values_to_rep(np.stack(truths)), dtype=torch.float64, device="cuda")
+    bf.b.conv_impl = 2
+    ref_m, ref_j = [t.clone() for t in bf.jacobian(rep)]
+    bf.b.conv_impl = 3
+    m, j = bf.jacobian(rep)
+    assert float((m - ref_m).abs().max() / ref_m.abs().max()) < 1e-6
+    scale = ref_j.abs().amax(dim=0)
+    assert float(((j - ref_j).abs().amax(dim=0) / scale).max()) < 1e-6
+    moffat = ap.utils.moffat_psf(3.0, 2.0, 11, 1.0)
+    mixed = BatchFit(cuts[:2] + [Cutout(np.zeros((40, 40)), np.ones((40, 40)), moffat)])
+    assert not mixed.sep_all
+    mixed.b.conv_impl = 3
+    rep3 = torch.tensor(values_to_rep(np.stack(truths[:2] + [truths[0]])), dtype=torch.float64, device="cuda")
+    with pytest.raises(ap.errors.APXError, match="conv_impl 3"):
+        mixed.render(rep3)

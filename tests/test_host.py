@@ -277,3 +277,17 @@ def test_upscaled_render_geometry(monkeypatch, U):
     assert (bs.scale, bs.nc, bs.w, bs.h) == (U, 1, nfx // U, nfy // U)
     assert (conv.x0, conv.y0, conv.ow, conv.oh) == (7, 7, U * bs.w, U * bs.h)
     assert (bs.dst_x0, bs.dst_y0, bs.dst_w, bs.dst_h) == (0, 0, 30, 26)
+
+
+def test_rank_one_factors():
+    """Separable-convolution eligibility: Gaussian PSFs (gaussian_psf, a
+    block-averaged product of 1-D Gaussians) factor to rounding; a Moffat PSF,
+    a zero PSF and a rank-two image do not."""
+    from astrophot_amd.batch import rank_one_factors
+    from astrophot_amd.utils import moffat_psf
+    gs = [H.gaussian_psf(s, 2 * math.ceil(4 * s) + 1, 1.0) for s in (1.0, 1.7, 2.9)]
+    rank2 = np.outer(np.arange(1.0, 8.0), np.ones(7)) + np.outer(np.ones(7), np.arange(7.0) ** 2)
+    f = rank_one_factors(gs + [moffat_psf(3.0, 2.0, 11, 1.0), np.zeros((5, 5)), rank2])
+    for p, uv in zip(gs, f[:3]):
+        assert uv is not None and np.abs(np.outer(*uv) - p).max() <= 1e-15 * p.max()
+    assert f[3:] == [None, None, None]

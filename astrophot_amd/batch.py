@@ -103,6 +103,7 @@ SIGNATURES = {
     "apx_batch_results": (ctypes.c_int, [ctypes.POINTER(Batch)] + [ctypes.c_void_p] * 7),
     "apx_batch_start_fallback": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
+    "apx_batch_counters": (ctypes.c_int, [ctypes.POINTER(Batch), ctypes.c_void_p, ctypes.c_void_p]),
 }
 L.register(SIGNATURES)
 
@@ -318,6 +319,19 @@ class BatchFit:
                   "quad_blocks": b.quad_blocks, "chunk_blocks": b.chunk_blocks}
         self.tuning = {"choice": choice, "timings_ms": log}
         return self.tuning
+
+    def counters(self) -> Dict[str, int]:
+        """Work counters of the last render / Jacobian stage (apx_batch_counters)."""
+        out = torch.zeros(8, dtype=torch.int32, device=self.data.device)
+        L.check(self.lib.apx_batch_counters(ctypes.byref(self.b), out.data_ptr(), L.stream()))
+        c = out.cpu().tolist()
+        return {"fits": c[0], "chunks": c[1], "tiles": c[2], "refined": c[3], "undecided_select": c[4],
+                "redone": c[5], "level3_float": c[6], "level3_double": c[7]}
+
+    def quad_evaluations(self, counts: Dict[str, int]) -> int:
+        """Profile evaluations of the float32 level-2/3 quadrature pass."""
+        per = (self.b.gridding ** 2) * (self.b.quad_level ** 2)
+        return per * (counts["refined"] + counts["level3_float"])
 
     def conv_flops(self) -> float:
         """Algorithmic flops of one value convolution of every cutout:

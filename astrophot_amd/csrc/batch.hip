@@ -708,6 +708,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
     }
   }
   const uint64_t gmask = GS == 64 ? ~0ull : (((1ull << GS) - 1ull) << (grp * GS));
+  int nsub3 = 0;  // level-3 sub-pixels this wave integrates (work counter, wave-uniform)
   for (int k0 = wave * NG; k0 < total; k0 += nwaves * NG) {
     const int k = k0 + grp;
     const bool active = k < total;
@@ -760,6 +761,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
     }
     // level 3: the group integrates each flagged sub-pixel in turn
     uint64_t todo = (__ballot(deeper) & gmask) >> (GS == 64 ? 0 : grp * GS);
+    nsub3 += (int)__popcll(__ballot(deeper));
     const float sc3 = ig * ig * ig * ig;
     while (__ballot(todo != 0ull)) {
       int sp = -1;
@@ -831,6 +833,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
       }
     }
   }
+  if (lane == 0 && nsub3 > 0) atomicAdd(&s.cnt[F64 ? 7 : 6], nsub3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1726,10 +1729,11 @@ QuadTab quad_table(const apx_batch& b) {
 
 template <int GS, int QN, int QG>
 void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac,
-                      hipStream_t stream) {
+                      hipStream_t stream, bool float_pass_only = false) {
   const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
+  if (float_pass_only) return;
   scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 5, s.bsum, 0, stream);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
@@ -1756,6 +1760,28 @@ void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, boo
   scan_counts(b.ngal, s.gal_cnt, s.gal_off, s.cnt + 3, s.bsum, 0, stream);
   if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32, QN, QG>(b, s, t, jac, stream);
   else launch_integrate<64, QN, QG>(b, s, t, jac, stream);
+}
+
+__global__ void zero_ints_kernel(int32_t* p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// the float32 level-2/3 quadrature pass of the previous render again, on its
+// deep list (for timing that kernel alone): the per-fit redo counts it appends
+// to start from zero again, and undecided pixels are left as the float64 pass
+// wrote them, so the planes do not change
+void quad_again(const apx_batch& b, const Scratch& s, bool jac, hipStream_t stream) {
+  const QuadTab t = quad_table(b);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((b.ngal + 255) / 256), dim3(256), 0, stream, s.gal_rcnt, b.ngal);
+  const bool g32 = t.gg <= 32 && b.quad_group != 64;
+  if (t.n == 3 && t.g == 5) {
+    if (g32) launch_integrate<32, 3, 5>(b, s, t, jac, stream, true);
+    else launch_integrate<64, 3, 5>(b, s, t, jac, stream, true);
+  } else {
+    if (g32) launch_integrate<32, 0, 0>(b, s, t, jac, stream, true);
+    else launch_integrate<64, 0, 0>(b, s, t, jac, stream, true);
+  }
 }
 
 // work lists + render planes of the fits active in `stage`
@@ -1814,6 +1840,10 @@ void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipSt
 // one render stage over the fits active in `stage`; parts: 1 planes, 2 conv
 int run_stage(const apx_batch& b, const Scratch& s, int stage, int mode, const double* rep_override,
               float* out, hipStream_t stream, int parts = 3) {
+  if (parts == 4) {
+    quad_again(b, s, stage == ST_JAC || stage == ST_ALL_JAC, stream);
+    return check_launch("batch stage");
+  }
   if (parts & 1) render_part(b, s, stage, rep_override, stream);
   if (parts & 2) conv_part(b, s, mode, out, stream, stage != ST_ALL && stage != ST_ALL_JAC);
   return check_launch("batch stage");
@@ -1896,7 +1926,7 @@ int batch_render(const apx_batch* b, const double* rep, float* out, int32_t part
   int rc = check_batch(b);
   if (rc) return rc;
   if (!rep || !out) return set_error(APX_E_INVALID, "batch_render: null pointer");
-  if (parts < 1 || parts > 3) return set_error(APX_E_INVALID, "batch_render: parts must be 1, 2 or 3");
+  if (parts < 1 || parts > 4) return set_error(APX_E_INVALID, "batch_render: parts must be 1, 2, 3 or 4");
   const Scratch s = carve(*b);
   return run_stage(*b, s, ST_ALL, EPI_OUT, rep, out, stream, parts);
 }
@@ -1906,7 +1936,7 @@ int batch_jacobian(const apx_batch* b, const double* rep, float* out, int32_t pa
   int rc = check_batch(b);
   if (rc) return rc;
   if (!rep || !out) return set_error(APX_E_INVALID, "batch_jacobian: null pointer");
-  if (parts < 1 || parts > 3) return set_error(APX_E_INVALID, "batch_jacobian: parts must be 1, 2 or 3");
+  if (parts < 1 || parts > 4) return set_error(APX_E_INVALID, "batch_jacobian: parts must be 1, 2, 3 or 4");
   const Scratch s = carve(*b);
   return run_stage(*b, s, ST_ALL_JAC, EPI_NORMAL, rep, out, stream, parts);
 }
@@ -1935,6 +1965,17 @@ int batch_tick(const apx_batch* b, hipStream_t stream) {
   if ((rc = run_stage(*b, s, ST_TRIAL, EPI_CHI2, nullptr, b->model1, stream))) return rc;
   hipLaunchKernelGGL(lm_decide_kernel, dim3(gb), dim3(256), 0, stream, *b);
   return check_launch("batch_tick");
+}
+
+__global__ void copy_ints_kernel(const int32_t* src, int32_t* dst, int n) {
+  if ((int)threadIdx.x < n) dst[threadIdx.x] = src[threadIdx.x];
+}
+
+int batch_counters(const apx_batch* b, int32_t* out, hipStream_t stream) {
+  if (!b || !b->scratch || !out) return set_error(APX_E_INVALID, "batch_counters: null pointer");
+  const Scratch s = carve(*b);
+  hipLaunchKernelGGL(copy_ints_kernel, dim3(1), dim3(64), 0, stream, s.cnt, out, 8);
+  return check_launch("batch_counters");
 }
 
 int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,

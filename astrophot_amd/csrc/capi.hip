@@ -181,6 +181,10 @@ int apx_batch_tick(const apx_batch* batch, void* stream) {
   return apx::batch_tick(batch, static_cast<hipStream_t>(stream));
 }
 
+int apx_batch_counters(const apx_batch* batch, int32_t* out, void* stream) {
+  return apx::batch_counters(batch, out, static_cast<hipStream_t>(stream));
+}
+
 int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
                       int32_t* iterations, int32_t* fallback, int32_t* done, void* stream) {
   return apx::batch_results(batch, rep, chi2, status, iterations, fallback, done,

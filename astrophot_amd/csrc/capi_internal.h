@@ -81,6 +81,7 @@ int colsum(const float* X, int M, int N, float* out, hipStream_t stream);
 int adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
          float eps, int step, uint16_t* wb, hipStream_t stream);
 int asinh_input(const float* x, const float* sigma, int Bn, int npx, uint16_t* out, hipStream_t stream);
+int batch_counters(const apx_batch* b, int32_t* out, hipStream_t stream);
 int batch_results(const apx_batch* b, double* rep, double* chi2, int32_t* status, int32_t* iters,
                   int32_t* fallback, int32_t* done, hipStream_t stream);
 

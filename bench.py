@@ -181,16 +181,16 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "roofline_traffic.json")
 
 
 def measured_traffic(kernel: str, ngal: int):
-    """HBM bytes per launch of the roofline kernel from the rocprofv3 --pmc
-    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/roofline_traffic.json
-    (calibrated against a device copy of known size), when they were taken on
-    the same kernel and workload size; else None."""
+    """HBM bytes per launch of ``kernel`` from the rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/roofline_traffic.json (calibrated
+    against a device copy of known size), when they were taken on the same
+    kernel and workload size; else None."""
     try:
         with open(TRAFFIC_JSON) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
+            t = json.load(f).get("kernels", {}).get(kernel)
+    except (OSError, ValueError, AttributeError):
         return None
-    if t.get("kernel") != kernel or t.get("ngal") != ngal:
+    if not t or t.get("ngal") != ngal:
         return None
     return t
 
@@ -208,11 +208,11 @@ def timed_events(fn, n=10, skip=2):
     return 1e3 * float(np.mean([a.elapsed_time(b) for a, b in evs[skip:]]))
 
 
-def roofline(bf, truth_rep, label):
-    """The dominant kernel: the Jacobian-stage convolution (value with K,
-    dK/dcx, dK/dcy and the q, PA, n, Re partial planes with K: 7 kernel passes
-    per output pixel; the Ie column is ln(10) times the model, one multiply)
-    of every cutout, timed live; the value convolution beside it."""
+def conv_roofline(bf, truth_rep, label):
+    """The Jacobian-stage convolution (value with K, dK/dcx, dK/dcy and the
+    q, PA, n, Re partial planes with K: 7 kernel passes per output pixel; the
+    Ie column is ln(10) times the model, one multiply) of every cutout, timed
+    live; the value convolution beside it."""
     out = torch.zeros(bf.npix, dtype=torch.float32, device="cuda")
     bf.jacobian(truth_rep, out=out, parts=1)
     jac_us = timed_events(lambda: bf.jacobian(truth_rep, out=out, parts=2))
@@ -249,6 +249,37 @@ def roofline(bf, truth_rep, label):
             "value_conv": {"kernel": kernel.replace("<1>", "<0>"), "algorithmic_per_launch": vflops,
                            "avg_launch_us": val_us,
                            "achieved": vflops / (val_us * 1e-6) / 1e12}}
+
+
+# flops of one float32 profile evaluation of the quadrature (sersic_f value
+# path: sample position 2, rotation and 1/q stretch 7, R^2 4, log2 and the
+# first scale 3, the 1/n power 2, amplitude exponential 5, weighted sum 2;
+# an FMA counts 2, a transcendental 1)
+FLOPS_PER_EVAL = 25.0
+
+
+def quad_roofline(bf, truth_rep, label):
+    """The dominant kernel: the float32 level-2/3 quadrature pass of every
+    refined pixel (utils/operations.py:229-247), re-run alone on the lists of
+    a render of every cutout and timed live; its work is the profile
+    evaluations it makes (the render's work counters) times FLOPS_PER_EVAL."""
+    out = torch.zeros(bf.npix, dtype=torch.float32, device="cuda")
+    bf.render(truth_rep, out=out, parts=1)
+    counts = bf.counters()
+    evals = bf.quad_evaluations(counts)
+    us = timed_events(lambda: bf.render(truth_rep, out=out, parts=4))
+    b = bf.b
+    gs = 32 if (b.gridding * b.gridding <= 32 and b.quad_group != 64) else 64
+    fixed = b.quad_level == 3 and b.gridding == 5
+    kernel = f"integrate_kernel<false, {gs}, false, {3 if fixed else 0}, {5 if fixed else 0}>"
+    flops = FLOPS_PER_EVAL * evals
+    rate = flops / (us * 1e-6)
+    t = measured_traffic(kernel, len(bf.gal_host))
+    return {"bound": "valu", "kernel": f"{kernel} (float32 level-2/3 quadrature of every refined pixel, {label})",
+            "achieved": rate / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s", "frac": rate / PEAK_FP32,
+            "traffic": t["bytes_per_launch"] if t else None, "traffic_detail": t,
+            "algorithmic_per_launch": flops, "evaluations_per_launch": evals,
+            "flops_per_evaluation": FLOPS_PER_EVAL, "avg_launch_us": us, "work_counters": counts}
 
 
 def agreement(values, chi2, plain, idx):
@@ -306,8 +337,9 @@ def c4_leg(args):
     start = rep_to_values(bf.moment_init().cpu().numpy())
     plain, t_plain = plain_fits(pop, cuts, data, var, start, idx)
     plain_fps = len(idx) / t_plain
-    roof = roofline(bf, torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda"),
-                    f"all {n} cutouts")
+    truth = torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda")
+    roof = quad_roofline(bf, truth, f"all {n} cutouts")
+    roof_conv = conv_roofline(bf, truth, f"all {n} cutouts")
     return {
         "fits_per_s": n / sec, "seconds_per_step": sec, "ngal": n, "total_pixels": int(bf.npix),
         "fallback_rate": float(res["fallback"].mean()),
@@ -328,6 +360,7 @@ def c4_leg(args):
         # outcome: where the learned pipeline ends against the plain path
         "learned_vs_plain_chi2": learned_vs_plain(res["chi2"], plain, idx),
         "roofline": roof,
+        "roofline_conv": roof_conv,
         "autotune": tuning["choice"],
     }, (pop, data, var, start)
 
@@ -339,6 +372,7 @@ def c2_leg(args):
     pop, cuts, data, var, start = c2_problem(args.ngal)
     bf = BatchFit(cuts, data=data, variance=var)
     rep0 = torch.tensor(values_to_rep(start), dtype=torch.float64, device="cuda")
+    tuning = bf.autotune(rep0)
     for _ in range(args.warmup):
         res = bf.fit(rep0)
     torch.cuda.synchronize()
@@ -358,8 +392,11 @@ def c2_leg(args):
         "plain_gpu": {"fits_per_s": len(idx) / t_plain, "cutouts": len(idx), "seconds": t_plain},
         "batched_vs_plain": (args.ngal / sec) / (len(idx) / t_plain),
         "batched_vs_plain_agreement": agreement(res["values"], res["chi2"], plain, idx),
-        "roofline": roofline(bf, torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda"),
-                             f"all {args.ngal} cutouts"),
+        "autotune": tuning["choice"],
+        "roofline": quad_roofline(bf, torch.tensor(values_to_rep(pop.truth), dtype=torch.float64, device="cuda"),
+                                  f"all {args.ngal} cutouts"),
+        "roofline_conv": conv_roofline(bf, torch.tensor(values_to_rep(pop.truth), dtype=torch.float64,
+                                                        device="cuda"), f"all {args.ngal} cutouts"),
     }
     return out, (pop, data, var, start)
 
@@ -420,8 +457,9 @@ def c3_leg(nsrc=500, side=2048, seed=SEED + 3):
 def kernel_only(args):
     """For counter collection (rocprofv3 --pmc): the headline batch's launch
     shape chosen by the autotuner, then its Jacobian-stage convolution over
-    every cutout 3 times (the last 3 launches of that kernel), then a plain
-    1 GiB device copy twice (the counters' calibration: a copy of known size)."""
+    every cutout 3 times and its float32 quadrature pass over every cutout 3
+    times (the last 3 launches of each kernel), then a plain 1 GiB device copy
+    twice (the counters' calibration: a copy of known size)."""
     from astrophot_amd import learned_init as LI
     from astrophot_amd.batch import BatchFit, values_to_rep
 
@@ -433,12 +471,16 @@ def kernel_only(args):
     bf.jacobian(rep, out=out, parts=1)
     for _ in range(3):
         bf.jacobian(rep, out=out, parts=2)
+    bf.render(rep, out=out, parts=1)
+    for _ in range(3):
+        bf.render(rep, out=out, parts=4)  # the float32 quadrature pass alone
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(2):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"kernel_only": True, "ngal": args.ngal4, "jacobian_conv_launches": 3, "choice": choice,
+    print(json.dumps({"kernel_only": True, "ngal": args.ngal4, "jacobian_conv_launches": 3,
+                      "quadrature_launches": 3, "choice": choice,
                       "copy_bytes": src.numel() * 4, "copies": 2}))
     return 0
 

@@ -319,7 +319,9 @@ int64_t apx_batch_scratch_bytes(const apx_batch* batch);
 
 /* Render every cutout at rep (device, ngal x 7 float64) into out (npix).
  * parts: 1 = render the working planes only, 2 = convolve planes rendered by
- * a previous parts=1 call, 3 = both. */
+ * a previous parts=1 call, 3 = both, 4 = the float32 level-2/3 quadrature
+ * pass of the previous parts=1 call again on its lists (planes unchanged;
+ * for timing that launch alone). */
 int apx_batch_render(const apx_batch* batch, const double* rep, float* out, int32_t parts,
                      void* stream);
 
@@ -369,6 +371,17 @@ int apx_batch_tick(const apx_batch* batch, void* stream);
  * finished fits in done[0].  Any output but rep may be null. */
 int apx_batch_results(const apx_batch* batch, double* rep, double* chi2, int32_t* status,
                       int32_t* iterations, int32_t* fallback, int32_t* done, void* stream);
+
+/* Work counters of the last render / Jacobian / tick stage, copied to out
+ * (device, 8 int32): [0] fits listed, [1] 256-pixel chunks, [2] output tiles,
+ * [3] pixels refined past level 1 (each integrates gridding^2 level-2
+ * sub-pixels of quad_level^2 samples), [4] undecided level-1 selections,
+ * [5] pixels redone in float64, [6] level-3 sub-pixels integrated by the
+ * float32 pass, [7] by the float64 pass (each quad_level^2 gridding^2
+ * samples).  The profile evaluations of the float32 level-2/3 pass are
+ * gridding^2 quad_level^2 ([3] + [6]).  No counterpart in the reference:
+ * it measures the work of utils/operations.py:210-247 as run here. */
+int apx_batch_counters(const apx_batch* batch, int32_t* out, void* stream);
 
 /* Last error message of the calling thread. */
 const char* apx_last_error(void);

@@ -253,3 +253,24 @@ def test_separable_convolution_sheared_pixels_and_refusal():
     rep3 = torch.tensor(values_to_rep(np.stack(truths[:2] + [truths[0]])), dtype=torch.float64, device="cuda")
     with pytest.raises(ap.errors.APXError, match="conv_impl 3"):
         mixed.render(rep3)
+
+
+def test_quadrature_rerun_and_work_counters():
+    """parts=4 re-runs the float32 level-2/3 quadrature pass on the previous
+    render's lists: the planes, and so the convolved render, stay bit-equal;
+    the work counters add that pass's level-3 sub-pixels once per run."""
+    truths, shapes, rng = make_cutouts(5, seed=17)
+    cuts = [Cutout(np.zeros((s, s)), np.ones((s, s)), psf) for s, psf in shapes]
+    bf = BatchFit(cuts)
+    rep = torch.tensor(values_to_rep(np.stack(truths)), dtype=torch.float64, device="cuda")
+    ref = bf.render(rep).clone()
+    c1 = bf.counters()
+    assert c1["fits"] == 5 and c1["refined"] > 0 and c1["tiles"] == int(bf.b.ntile)
+    assert bf.quad_evaluations(c1) == 225 * (c1["refined"] + c1["level3_float"])
+    for _ in range(3):
+        bf.render(rep, parts=4)
+    c2 = bf.counters()
+    assert c2["refined"] == c1["refined"] and c2["level3_float"] == 4 * c1["level3_float"]
+    out = torch.zeros_like(ref)
+    bf.render(rep, out=out, parts=2)
+    assert torch.equal(out, ref)

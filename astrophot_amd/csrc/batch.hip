@@ -79,6 +79,7 @@ struct Scratch {
   int32_t* gal_cnt;   // ngal: pixels each fit sends to the level-2 quadrature
   int32_t* gal_off;   // ngal + 1: exclusive prefix of gal_cnt
   int32_t* deep;      // nwork: per-fit segments (at work_off) of those pixels
+  int2* dq;           // nwork: the same list after the scan, (fit, pixel) in global order
   int32_t* gal_rcnt;  // ngal: pixels each fit sends to a float64 redo pass
   int32_t* gal_roff;  // ngal + 1: exclusive prefix of gal_rcnt
   int32_t* redo;      // nwork: per-fit segments of those pixels
@@ -110,6 +111,7 @@ __host__ inline int64_t scratch_layout(const apx_batch& b, Scratch* s) {
   t.gal_cnt = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
   t.gal_off = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
   t.deep = reinterpret_cast<int32_t*>(take(4 * b.nwork));
+  t.dq = reinterpret_cast<int2*>(take(8 * b.nwork));
   t.gal_rcnt = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
   t.gal_roff = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
   t.redo = reinterpret_cast<int32_t*>(take(4 * b.nwork));
@@ -714,8 +716,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
     const bool active = k < total;
     int gal = 0, pix = 0;
     if (active) {
-      gal = locate(off, b.ngal, k);
-      pix = list[b.gal[gal].work_off + (k - off[gal])];
+      if (F64) {
+        gal = locate(off, b.ngal, k);
+        pix = list[b.gal[gal].work_off + (k - off[gal])];
+      } else {
+        const int2 q = s.dq[k];
+        gal = q.x;
+        pix = q.y;
+      }
     }
     const GalEval* E = s.ev + gal;  // fit 0's constants for an idle group: read, never used
     const Prof<float> pf = E->pf;
@@ -1727,6 +1735,18 @@ QuadTab quad_table(const apx_batch& b) {
   return t;
 }
 
+// the scanned deep list as (fit, pixel) pairs in global order, one wave per
+// fit, so the quadrature reads each entry's fit instead of searching the offsets
+__global__ __launch_bounds__(256) void compact_deep_kernel(apx_batch b, Scratch s) {
+  const int lane = threadIdx.x & 63;
+  for (int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6); g < b.ngal; g += gridDim.x * 4) {
+    const int a = s.gal_off[g], n = s.gal_off[g + 1] - a;
+    if (n == 0) continue;
+    const int32_t* src = s.deep + b.gal[g].work_off;
+    for (int j = lane; j < n; j += 64) s.dq[a + j] = make_int2(g, src[j]);
+  }
+}
+
 template <int GS, int QN, int QG>
 void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac,
                       hipStream_t stream, bool float_pass_only = false) {
@@ -1758,6 +1778,7 @@ void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, boo
   if (jac) hipLaunchKernelGGL((select_redo_kernel<true, QN>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((select_redo_kernel<false, QN>), ig, dim3(256), 0, stream, b, s, t);
   scan_counts(b.ngal, s.gal_cnt, s.gal_off, s.cnt + 3, s.bsum, 0, stream);
+  hipLaunchKernelGGL(compact_deep_kernel, dim3(std::min((b.ngal + 3) / 4, 4096)), dim3(256), 0, stream, b, s);
   if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32, QN, QG>(b, s, t, jac, stream);
   else launch_integrate<64, QN, QG>(b, s, t, jac, stream);
 }

@@ -80,6 +80,7 @@ struct Scratch {
   int32_t* gal_off;   // ngal + 1: exclusive prefix of gal_cnt
   int32_t* deep;      // nwork: per-fit segments (at work_off) of those pixels
   int2* dq;           // nwork: the same list after the scan, (fit, pixel) in global order
+  int2* rq;           // nwork: the scanned redo list in the same form
   int32_t* gal_rcnt;  // ngal: pixels each fit sends to a float64 redo pass
   int32_t* gal_roff;  // ngal + 1: exclusive prefix of gal_rcnt
   int32_t* redo;      // nwork: per-fit segments of those pixels
@@ -112,6 +113,7 @@ __host__ inline int64_t scratch_layout(const apx_batch& b, Scratch* s) {
   t.gal_off = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
   t.deep = reinterpret_cast<int32_t*>(take(4 * b.nwork));
   t.dq = reinterpret_cast<int2*>(take(8 * b.nwork));
+  t.rq = reinterpret_cast<int2*>(take(8 * b.nwork));
   t.gal_rcnt = reinterpret_cast<int32_t*>(take(4 * (int64_t)b.ngal));
   t.gal_roff = reinterpret_cast<int32_t*>(take(4 * ((int64_t)b.ngal + 1)));
   t.redo = reinterpret_cast<int32_t*>(take(4 * b.nwork));
@@ -542,18 +544,6 @@ __device__ __forceinline__ void wave_append(bool flag, int32_t* counter, int32_t
   }
 }
 
-// fit owning entry k of a list whose per-fit segments start at off[g]
-// (exclusive prefix, off[ngal] = total > k)
-__device__ __forceinline__ int locate(const int32_t* off, int ngal, int k) {
-  int lo = 0, hi = ngal;  // off[lo] <= k < off[hi]
-  while (hi - lo > 1) {
-    const int md = (lo + hi) >> 1;
-    if (off[md] <= k) lo = md;
-    else hi = md;
-  }
-  return lo;
-}
-
 // float32 selection of every listed chunk, one lane per pixel
 template <bool GRAD, int QN>
 __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, QuadTab t) {
@@ -574,9 +564,9 @@ template <bool GRAD, int QN>
 __global__ __launch_bounds__(256) void select_redo_kernel(apx_batch b, Scratch s, QuadTab t) {
   const int total = s.cnt[4];
   for (int k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
-    const int g = locate(s.gal_roff, b.ngal, k);
+    const int2 q = s.rq[k];
+    const int g = q.x, pix = q.y;
     const apx_batch_galaxy G = b.gal[g];
-    const int pix = s.redo[G.work_off + (k - s.gal_roff[g])];
     if (select_pixel<GRAD, true, QN>(b, s, G, s.ev[g], t, pix) == PX_DEEPER)
       s.deep[G.work_off + atomicAdd(&s.gal_cnt[g], 1)] = pix;
   }
@@ -685,8 +675,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
   const int lane = threadIdx.x & 63, gl = lane % GS, grp = lane / GS;
   const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
   const int total = F64 ? s.cnt[5] : s.cnt[3];
-  const int32_t* off = F64 ? s.gal_roff : s.gal_off;
-  const int32_t* list = F64 ? s.redo : s.deep;
   const int qn = QN ? QN : t.n, qnn = qn * qn;
   const int g_ = QG ? QG : t.g, gg = g_ * g_, nq = gg * qnn;
   const float ig = 1.0f / (float)g_;
@@ -716,14 +704,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
     const bool active = k < total;
     int gal = 0, pix = 0;
     if (active) {
-      if (F64) {
-        gal = locate(off, b.ngal, k);
-        pix = list[b.gal[gal].work_off + (k - off[gal])];
-      } else {
-        const int2 q = s.dq[k];
-        gal = q.x;
-        pix = q.y;
-      }
+      const int2 q = F64 ? s.rq[k] : s.dq[k];
+      gal = q.x;
+      pix = q.y;
     }
     const GalEval* E = s.ev + gal;  // fit 0's constants for an idle group: read, never used
     const Prof<float> pf = E->pf;
@@ -1735,16 +1718,24 @@ QuadTab quad_table(const apx_batch& b) {
   return t;
 }
 
-// the scanned deep list as (fit, pixel) pairs in global order, one wave per
-// fit, so the quadrature reads each entry's fit instead of searching the offsets
-__global__ __launch_bounds__(256) void compact_deep_kernel(apx_batch b, Scratch s) {
+// a scanned per-fit list (segments at work_off, exclusive prefix off) as
+// (fit, pixel) pairs in global order, one wave per fit, so the passes over it
+// read each entry's fit instead of searching the offsets
+__global__ __launch_bounds__(256) void compact_list_kernel(apx_batch b, const int32_t* off,
+                                                           const int32_t* list, int2* out) {
   const int lane = threadIdx.x & 63;
   for (int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6); g < b.ngal; g += gridDim.x * 4) {
-    const int a = s.gal_off[g], n = s.gal_off[g + 1] - a;
+    const int a = off[g], n = off[g + 1] - a;
     if (n == 0) continue;
-    const int32_t* src = s.deep + b.gal[g].work_off;
-    for (int j = lane; j < n; j += 64) s.dq[a + j] = make_int2(g, src[j]);
+    const int32_t* src = list + b.gal[g].work_off;
+    for (int j = lane; j < n; j += 64) out[a + j] = make_int2(g, src[j]);
   }
+}
+
+inline void compact_list(const apx_batch& b, const int32_t* off, const int32_t* list, int2* out,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(compact_list_kernel, dim3(std::min((b.ngal + 3) / 4, 4096)), dim3(256), 0, stream, b, off,
+                     list, out);
 }
 
 template <int GS, int QN, int QG>
@@ -1755,6 +1746,7 @@ void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bo
   else hipLaunchKernelGGL((integrate_kernel<false, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   if (float_pass_only) return;
   scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 5, s.bsum, 0, stream);
+  compact_list(b, s.gal_roff, s.redo, s.rq, stream);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
 }
@@ -1775,10 +1767,11 @@ void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, boo
   // the deep list, then its undecided pixels in float64
   const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
   scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 4, s.bsum, 1, stream);
+  compact_list(b, s.gal_roff, s.redo, s.rq, stream);
   if (jac) hipLaunchKernelGGL((select_redo_kernel<true, QN>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((select_redo_kernel<false, QN>), ig, dim3(256), 0, stream, b, s, t);
   scan_counts(b.ngal, s.gal_cnt, s.gal_off, s.cnt + 3, s.bsum, 0, stream);
-  hipLaunchKernelGGL(compact_deep_kernel, dim3(std::min((b.ngal + 3) / 4, 4096)), dim3(256), 0, stream, b, s);
+  compact_list(b, s.gal_off, s.deep, s.dq, stream);
   if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32, QN, QG>(b, s, t, jac, stream);
   else launch_integrate<64, QN, QG>(b, s, t, jac, stream);
 }

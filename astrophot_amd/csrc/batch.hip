@@ -1721,10 +1721,12 @@ QuadTab quad_table(const apx_batch& b) {
 // a scanned per-fit list (segments at work_off, exclusive prefix off) as
 // (fit, pixel) pairs in global order, one wave per fit, so the passes over it
 // read each entry's fit instead of searching the offsets
-__global__ __launch_bounds__(256) void compact_list_kernel(apx_batch b, const int32_t* off,
+__global__ __launch_bounds__(256) void compact_list_kernel(apx_batch b, Scratch s, const int32_t* off,
                                                            const int32_t* list, int2* out) {
   const int lane = threadIdx.x & 63;
-  for (int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6); g < b.ngal; g += gridDim.x * 4) {
+  // only the fits listed for the stage have entries (build_kernel clears every count)
+  for (int i = blockIdx.x * 4 + (int)(threadIdx.x >> 6); i < s.cnt[0]; i += gridDim.x * 4) {
+    const int g = s.gal_list[i];
     const int a = off[g], n = off[g + 1] - a;
     if (n == 0) continue;
     const int32_t* src = list + b.gal[g].work_off;
@@ -1732,10 +1734,10 @@ __global__ __launch_bounds__(256) void compact_list_kernel(apx_batch b, const in
   }
 }
 
-inline void compact_list(const apx_batch& b, const int32_t* off, const int32_t* list, int2* out,
-                         hipStream_t stream) {
-  hipLaunchKernelGGL(compact_list_kernel, dim3(std::min((b.ngal + 3) / 4, 4096)), dim3(256), 0, stream, b, off,
-                     list, out);
+inline void compact_list(const apx_batch& b, const Scratch& s, const int32_t* off, const int32_t* list,
+                         int2* out, hipStream_t stream) {
+  hipLaunchKernelGGL(compact_list_kernel, dim3(std::min((b.ngal + 3) / 4, 4096)), dim3(256), 0, stream, b, s,
+                     off, list, out);
 }
 
 template <int GS, int QN, int QG>
@@ -1746,7 +1748,7 @@ void launch_integrate(const apx_batch& b, const Scratch& s, const QuadTab& t, bo
   else hipLaunchKernelGGL((integrate_kernel<false, GS, false, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   if (float_pass_only) return;
   scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 5, s.bsum, 0, stream);
-  compact_list(b, s.gal_roff, s.redo, s.rq, stream);
+  compact_list(b, s, s.gal_roff, s.redo, s.rq, stream);
   if (jac) hipLaunchKernelGGL((integrate_kernel<true, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((integrate_kernel<false, GS, true, QN, QG>), ig, dim3(256), 0, stream, b, s, t);
 }
@@ -1767,11 +1769,11 @@ void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, boo
   // the deep list, then its undecided pixels in float64
   const dim3 ig(b.quad_blocks > 0 ? b.quad_blocks : kPersist / 2);
   scan_counts(b.ngal, s.gal_rcnt, s.gal_roff, s.cnt + 4, s.bsum, 1, stream);
-  compact_list(b, s.gal_roff, s.redo, s.rq, stream);
+  compact_list(b, s, s.gal_roff, s.redo, s.rq, stream);
   if (jac) hipLaunchKernelGGL((select_redo_kernel<true, QN>), ig, dim3(256), 0, stream, b, s, t);
   else hipLaunchKernelGGL((select_redo_kernel<false, QN>), ig, dim3(256), 0, stream, b, s, t);
   scan_counts(b.ngal, s.gal_cnt, s.gal_off, s.cnt + 3, s.bsum, 0, stream);
-  compact_list(b, s.gal_off, s.deep, s.dq, stream);
+  compact_list(b, s, s.gal_off, s.deep, s.dq, stream);
   if (t.gg <= 32 && b.quad_group != 64) launch_integrate<32, QN, QG>(b, s, t, jac, stream);
   else launch_integrate<64, QN, QG>(b, s, t, jac, stream);
 }

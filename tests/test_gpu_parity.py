@@ -392,3 +392,28 @@ def test_masked_member_keeps_other_light():
     # group Jacobian: the masked member's columns vanish at its masked pixels
     J = grp.jacobian().data
     assert float(J[10, 13, 1:].abs().max()) == 0.0 and J[10, 13, 0].item() != 0.0
+
+
+def test_group_with_upscaled_psf():
+    """Sersic + flat sky group on a target whose PSF is 3 times finer
+    (target_image.py:303-324): the group image and Jacobian against the
+    oracle; the sky ignores the PSF."""
+    nx, ny, U = 42, 38, 3
+    psf = H.gaussian_psf(1.2, 15, 1.0 / U)
+    tgt = ap.image.Target_Image(data=np.zeros((ny, nx)), pixelscale=1.0, psf=psf, psf_upscale=U)
+    p = {"center": [21.3, 18.6], "q": 0.7, "PA": 0.6, "n": 2.0, "Re": 6.0, "Ie": 1.0}
+    oc = H.Component("sersic", H.Grid(nx, ny), dict(p), psf=torch.tensor(psf), psf_mode="full", psf_upscale=U)
+    osky = H.Component("flat_sky", H.Grid(nx, ny), {"F": math.log10(0.5)})
+    om = H.Model([oc, osky], window=H.Grid(nx, ny))
+    ms = ap.models.AstroPhot_Model(name="s_up", model_type="sersic galaxy model", target=tgt, parameters=dict(p),
+                                   psf_mode="full")
+    mk = ap.models.AstroPhot_Model(name="sky_up", model_type="flat sky model", target=tgt,
+                                   parameters={"F": math.log10(0.5)})
+    g = ap.models.AstroPhot_Model(name="g_up", model_type="group model", models=[ms, mk], target=tgt)
+    rep = om.vector_representation()
+    assert np.allclose(rep.numpy(), g.parameters.vector_representation().numpy())
+    assert rel_err(g().data, om.forward(rep)) < IMG_TOL
+    J = g.jacobian(parameters=rep.clone(), as_representation=True).data
+    Jref = om.jacobian(rep)
+    for k in range(J.shape[-1]):
+        assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k

@@ -178,3 +178,22 @@ def test_lm_fits_galaxy_and_psf_width():
     assert abs(float(mpsf2["Rd"].value) - 1.8) < 1e-3
     assert abs(float(fit_m["Re"].value) - 6.0) < 1e-3
     assert np.allclose(fit_m["center"].value.numpy(), [20.3, 17.6], atol=1e-4)
+
+
+def test_galaxy_with_upscaled_psf_model():
+    """An auxiliary Moffat_PSF on a PSF window twice as fine as the image
+    (model_object.py:313-314 take the upscale from the PSF's pixel length):
+    image and the 9-column Jacobian (n, Rd first) against the oracle."""
+    opsf, mpsf = psf_pair(2.2, 1.8, 21, P=0.5)
+    og = H.Component("sersic", H.Grid(40, 36), dict(SERSIC), psf=opsf, psf_mode="full")
+    om = H.Model([og])
+    mg = ap.models.AstroPhot_Model(name="gal_up", model_type="sersic galaxy model", target=target(40, 36), psf=mpsf,
+                                   psf_mode="full", parameters=dict(SERSIC))
+    rep = om.vector_representation()
+    assert np.allclose(rep.numpy(), mg.parameters.vector_representation().numpy())
+    assert rel_err(mg().data, om.forward(rep)) < IMG_TOL
+    Jref = om.jacobian(rep)
+    J = mg.jacobian(parameters=rep.clone(), as_representation=True).data
+    assert J.shape == Jref.shape == (36, 40, 9)
+    for k in range(J.shape[-1]):
+        assert rel_err(J[..., k], Jref[..., k]) < JAC_TOL, k

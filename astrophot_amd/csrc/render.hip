@@ -208,11 +208,12 @@ __global__ __launch_bounds__(256) void render_integrate_kernel(apx_component cp,
   const int count = ws.count[0];
   const int64_t npix = (int64_t)cp.nx * cp.ny;
   const Prof<double> p = make_prof<double>(cp);
+  const Prof<float> pf = make_prof<float>(cp);
   const double thr1 = cp.tolerance * (cp.mean_reference ? ws.reference[0] : cp.reference);
   for (int k = wave; k < count; k += nwaves) {
     const int32_t pix = ws.list[k];
     double out[1 + APX_NSLOT];
-    integrate_pixel<GRAD>(cp, p, cp.nx, cp.ci, cp.cj, pix, thr1, lane, out);
+    integrate_pixel<GRAD>(cp, p, pf, cp.nx, cp.ci, cp.cj, pix, thr1, lane, out);
     if (lane == 0) {
       planes[pix] = (float)out[0];
       if (GRAD) {
@@ -274,6 +275,7 @@ __global__ __launch_bounds__(256) void render_vjp_integrate_kernel(apx_component
   const int nwaves = (int)(gridDim.x * 4);
   const int count = ws.count[0];
   const Prof<double> p = make_prof<double>(cp);
+  const Prof<float> pf = make_prof<float>(cp);
   const double thr1 = cp.tolerance * (cp.mean_reference ? ws.reference[0] : cp.reference);
   double acc[8];
 #pragma unroll
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(256) void render_vjp_integrate_kernel(apx_component
   for (int k = wave; k < count; k += nwaves) {
     const int32_t pix = ws.list[k];
     double out[1 + APX_NSLOT];
-    integrate_pixel<true>(cp, p, cp.nx, cp.ci, cp.cj, pix, thr1, lane, out);
+    integrate_pixel<true>(cp, p, pf, cp.nx, cp.ci, cp.cj, pix, thr1, lane, out);
     const double w = (double)B[pix];
     for (int c = 1; c < cp.nchan; ++c) acc[c] += w * out[1 + cp.chan_slot[c]];
   }

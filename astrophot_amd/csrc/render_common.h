@@ -60,11 +60,21 @@ __device__ T quad_sample(const Prof<T>& p, const apx_component& cp, T X, T Y, in
 }
 
 
-// float64 adaptive quadrature of one pixel by one wavefront (all lanes get the
+// Relative band around a threshold inside which a float32 test is taken again
+// in float64 (float32 profile samples carry far less relative error).
+constexpr double kSampleBand = 1e-4;
+
+// Adaptive quadrature of one pixel by one wavefront (all lanes get the
 // result): out[0] = value, out[1 + slot] = d value / d slot when GRAD.
+// Level 1, the value of every selected pixel that is not refined further, is
+// sampled in float64.  Levels 2 and 3 are sampled in float32 and
+// summed in float64; a level-2 test whose float32 value lies within
+// kSampleBand of its threshold is taken again from float64 samples, so the
+// sub-pixels refined are the ones float64 arithmetic picks.
 template <bool GRAD>
-__device__ void integrate_pixel(const apx_component& cp, const Prof<double>& p, int nx, int ci,
-                                int cj, int32_t pix, double thr1, int lane, double* out) {
+__device__ void integrate_pixel(const apx_component& cp, const Prof<double>& p, const Prof<float>& pf,
+                                int nx, int ci, int cj, int32_t pix, double thr1, int lane,
+                                double* out) {
   const int n = cp.quad_level, nn = n * n, g = cp.gridding, gg = g * g;
   const double ig = 1.0 / (double)g;
   const double sub0 = -(double)(g - 1) / (2.0 * g), subd = 1.0 / (double)g;  // linspace step
@@ -75,7 +85,7 @@ __device__ void integrate_pixel(const apx_component& cp, const Prof<double>& p, 
 #pragma unroll
   for (int s = 0; s <= APX_NSLOT; ++s) acc[s] = 0.0;
   double gl[APX_NSLOT];
-  // ---- level 1 (area 1): lane < nn is a node; node offsets are P / 2 * abscissa
+  // ---- level 1 (area 1, float64): lane < nn is a node; node offsets are P / 2 * abscissa
   double v1 = 0.0;
   if (lane < nn) {
     const double u = cp.gl_node[lane % n], v = cp.gl_node[lane / n];
@@ -90,7 +100,8 @@ __device__ void integrate_pixel(const apx_component& cp, const Prof<double>& p, 
   }
   const double res1 = wave_sum(acc[0]);
   const double ref1 = __shfl(v1, nn / 2, 64);
-  if (cp.max_depth <= 1 || !(fabs(res1 - ref1) > thr1)) {
+  const bool refine = cp.max_depth > 1 && fabs(res1 - ref1) > thr1;
+  if (!refine) {
     out[0] = res1;
     if (GRAD) {
 #pragma unroll
@@ -109,16 +120,29 @@ __device__ void integrate_pixel(const apx_component& cp, const Prof<double>& p, 
     const double a = sub0 + subd * (lane % g), b = sub0 + subd * (lane / g);
     x2 = X0 + p.P00 * a + p.P01 * b;
     y2 = Y0 + p.P10 * a + p.P11 * b;
-    double gs[APX_NSLOT], centre = 0.0;
-    const double r2 = quad_sample<double, GRAD>(p, cp, x2, y2, n, cp.gl_node, cp.gl_weight, sc2,
-                                                0.5 * ig * p.P00, 0.5 * ig * p.P01,
-                                                0.5 * ig * p.P10, 0.5 * ig * p.P11, gs, centre);
-    deeper = cp.max_depth > 2 && fabs(r2 - centre) > thr2;
+    float gs[APX_NSLOT], centre = 0.0f;
+    const float h = (float)(0.5 * ig);
+    const float r2 = quad_sample<float, GRAD>(pf, cp, (float)x2, (float)y2, n, cp.gl_node, cp.gl_weight,
+                                              (float)sc2, h * pf.P00, h * pf.P01, h * pf.P10, h * pf.P11,
+                                              gs, centre);
+    if (cp.max_depth > 2) {
+      const double D = fabs((double)r2 - (double)centre);
+      const double band = kSampleBand * (fabs((double)r2) + fabs((double)centre)) + 1e-7 * thr2;
+      if (D > thr2 + band) {
+        deeper = true;
+      } else if (D >= thr2 - band) {
+        double cd = 0.0;
+        const double rd = quad_sample<double, false>(p, cp, x2, y2, n, cp.gl_node, cp.gl_weight, sc2,
+                                                     0.5 * ig * p.P00, 0.5 * ig * p.P01,
+                                                     0.5 * ig * p.P10, 0.5 * ig * p.P11, nullptr, cd);
+        deeper = fabs(rd - cd) > thr2;
+      }
+    }
     if (!deeper) {
-      c2[0] = r2;
+      c2[0] = (double)r2;
       if (GRAD) {
 #pragma unroll
-        for (int s = 0; s < APX_NSLOT; ++s) c2[1 + s] = gs[s];
+        for (int s = 0; s < APX_NSLOT; ++s) c2[1 + s] = (double)gs[s];
       }
     }
   }
@@ -143,11 +167,12 @@ __device__ void integrate_pixel(const apx_component& cp, const Prof<double>& p, 
       const double hu = 0.5 * ig * ig * u, hv = 0.5 * ig * ig * v;
       const double x = xs + p.P00 * (a + hu) + p.P01 * (b + hv);
       const double y = ys + p.P10 * (a + hu) + p.P11 * (b + hv);
-      const double val = eval_profile<double, GRAD>(p, x, y, sc3, gl);
-      c3[0] += w * val;
+      float glf[APX_NSLOT];
+      const float val = eval_profile<float, GRAD>(pf, (float)x, (float)y, (float)sc3, glf);
+      c3[0] += w * (double)val;
       if (GRAD) {
 #pragma unroll
-        for (int s = 0; s < APX_NSLOT; ++s) c3[1 + s] += w * gl[s];
+        for (int s = 0; s < APX_NSLOT; ++s) c3[1 + s] += w * (double)glf[s];
       }
     }
 #pragma unroll

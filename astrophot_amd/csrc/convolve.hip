@@ -41,25 +41,38 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
   return red[0] + red[1] + red[2] + red[3];
 }
 
-__global__ __launch_bounds__(256) void psf_shift_kernel(const float* psf, int ph, int pw, double sx,
-                                                        double sy, double p0, double p1, double p2,
-                                                        double p3, double dvx, double dvy,
-                                                        float* out) {
-  __shared__ double red[4];
+// sum over a 1024-thread workgroup (every thread gets it)
+__device__ __forceinline__ double block_sum1024(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) t += red[w];
+  return t;
+}
+
+__global__ __launch_bounds__(1024) void psf_shift_kernel(const float* psf, int ph, int pw, double sx,
+                                                         double sy, double p0, double p1, double p2,
+                                                         double p3, double dvx, double dvy,
+                                                         float* out) {
+  __shared__ double red[16];
   const int H2 = ph + 2, W2 = pw + 2, n = H2 * W2;
   double S = 0.0, Sx = 0.0, Sy = 0.0;
-  for (int k = threadIdx.x; k < n; k += 256) {
+  for (int k = threadIdx.x; k < n; k += 1024) {
     double v, dx, dy;
     bilinear(psf, ph, pw, k / W2, k % W2, sx, sy, v, dx, dy);
     S += v;
     Sx += dx;
     Sy += dy;
   }
-  S = block_sum256(S, red);
-  Sx = block_sum256(Sx, red);
-  Sy = block_sum256(Sy, red);
+  S = block_sum1024(S, red);
+  Sx = block_sum1024(Sx, red);
+  Sy = block_sum1024(Sy, red);
   const double iS = 1.0 / S;
-  for (int k = threadIdx.x; k < n; k += 256) {
+  for (int k = threadIdx.x; k < n; k += 1024) {
     const int r = k / W2, c = k % W2;
     double v, dx, dy;
     bilinear(psf, ph, pw, r, c, sx, sy, v, dx, dy);
@@ -287,7 +300,7 @@ int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, co
     return set_error(APX_E_INVALID, "psf_shift: psf must have odd shape");
   if (ph + 2 > APX_MAX_KERNEL || pw + 2 > APX_MAX_KERNEL)
     return set_error(APX_E_LIMIT, "psf_shift: psf larger than APX_MAX_KERNEL - 2");
-  hipLaunchKernelGGL(psf_shift_kernel, dim3(1), dim3(256), 0, stream, psf, ph, pw, sx, sy, pinv[0],
+  hipLaunchKernelGGL(psf_shift_kernel, dim3(1), dim3(1024), 0, stream, psf, ph, pw, sx, sy, pinv[0],
                      pinv[1], pinv[2], pinv[3], dvdr_x, dvdr_y, kernels);
   return check_launch("psf_shift");
 }

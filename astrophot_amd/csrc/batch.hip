@@ -839,7 +839,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void i
 //   EPI_CHI2    sum W (Y - m)^2 of the tile                 (lm.py:290-292)
 enum : int { EPI_OUT = 0, EPI_NORMAL = 1, EPI_RPP = 2, EPI_CHI2 = 3 };
 
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 // dynamic LDS of conv_kernel: the staged input tile, reused for the
 // cross-wave reduction of one kernel's outputs at a time
@@ -984,58 +983,9 @@ __global__ __launch_bounds__(256) void conv_kernel(apx_batch b, Scratch s, float
 }
 
 // ---------------------------------------------------------------------------
-// MFMA convolution (kernels up to kMfmaMax): each wave computes a 16 x 16
-// block of the 64 x kTR output tile with v_mfma_f32_16x16x4f32 as
-//   out[r][x] = sum_kx sum_j A_kx[x][j] B_kx[j][r],
-//   A_kx[x][j] = in[j][x + kx],  B_kx[j][r] = Kf[j - r][kx] (zero outside the kernel),
-// j over the staged input rows in k-steps of 4.  A comes from the staged input
-// tile, B from a per-kernel tap table tab[kx][dd] = Kf[dd - 15][kx], both in
-// LDS.  Lane layout: A[l % 16][l / 16], B[l / 16][l % 16], and
-// D[4 (l / 16) + i][l % 16] in acc[i] (x = 4 (l / 16) + i, r = l % 16).
-// A kernel column needs kTR + kh - 1 input rows, ceil of that / 4 k-steps
-// (a compile-time count per tile, dispatched on the PSF size); the staged rows
-// and the tap table are padded with zeros to whole k-steps.
-__host__ __device__ inline int mf_nks(int kh) { return (kTR + kh - 1 + 3) / 4; }  // k-steps per kx
-__host__ __device__ inline int mf_rows(int kh) { return 4 * mf_nks(kh); }
-// row pitch of the staged input: 16 mod 32 floats, so the 4 rows x 16 columns
-// of an A operand (lanes l / 16, l % 16) fall in distinct LDS banks
-__host__ __device__ inline int mf_lw(int kw) { return 16 + 32 * ((64 + kw - 1 - 16 + 31) / 32); }
-__host__ __device__ inline int mf_dd(int kh) { return mf_rows(kh) + 15; }
-__host__ inline size_t mfma_lds_bytes(int kmax, int nk) {
-  return sizeof(float) * ((size_t)mf_rows(kmax) * mf_lw(kmax) + (size_t)nk * kmax * mf_dd(kmax));
-}
-
-// All kernel columns of one output block: column kx needs NKS k-steps; the
-// operands of the next column are loaded while the MFMAs of the current one
-// issue (two register sets of NKS A and B values).
-template <int NKS>
-__device__ __forceinline__ v4f mfma_columns(const float* ta, const float* tb, int kw, int LW, int DD) {
-  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  float a0[NKS], b0[NKS], a1[NKS], b1[NKS];
-  auto load = [&](int kx, float (&a)[NKS], float (&bb)[NKS]) {
-#pragma unroll
-    for (int t = 0; t < NKS; ++t) {
-      a[t] = ta[kx + 4 * t * LW];
-      bb[t] = tb[kx * DD + 4 * t];
-    }
-  };
-  load(0, a0, b0);
-  int kx = 0;
-  for (; kx + 1 < kw; kx += 2) {
-    load(kx + 1, a1, b1);
-#pragma unroll
-    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
-    if (kx + 2 < kw) load(kx + 2, a0, b0);
-#pragma unroll
-    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[t], b1[t], acc, 0, 0, 0);
-  }
-  if (kx < kw) {
-#pragma unroll
-    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
-  }
-  return acc;
-}
-
+// MFMA convolution (kernels up to kMfmaMax): the Toeplitz formulation of
+// conv_common.h (mfma_columns), one 16 x 16 output block per wave of the
+// 64 x kTR tile.
 template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void conv_mfma_kernel(apx_batch b, Scratch s, float* out) {
   extern __shared__ float smem[];
@@ -1096,18 +1046,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
       for (int kk = 0; kk < nk; ++kk) {
         const float* ta = tin + lk * LW + 16 * w + lm;
         const float* tb = tab + kk * kw * DD + lk - lm + 15;
-        v4f acc;
-        switch (nks) {  // compile-time k-step count per kernel column (kh <= kMfmaMax)
-          case 5: acc = mfma_columns<5>(ta, tb, kw, LW, DD); break;
-          case 6: acc = mfma_columns<6>(ta, tb, kw, LW, DD); break;
-          case 7: acc = mfma_columns<7>(ta, tb, kw, LW, DD); break;
-          case 8: acc = mfma_columns<8>(ta, tb, kw, LW, DD); break;
-          case 9: acc = mfma_columns<9>(ta, tb, kw, LW, DD); break;
-          case 10: acc = mfma_columns<10>(ta, tb, kw, LW, DD); break;
-          case 11: acc = mfma_columns<11>(ta, tb, kw, LW, DD); break;
-          case 12: acc = mfma_columns<12>(ta, tb, kw, LW, DD); break;
-          default: acc = mfma_columns<13>(ta, tb, kw, LW, DD); break;
-        }
+        const v4f acc = mfma_block(nks, ta, tb, kw, LW, DD);
         const int col = c == 0 ? kk - 1 : c + 1;  // J column, -1 for the value
 #pragma unroll
         for (int i = 0; i < 4; ++i) {

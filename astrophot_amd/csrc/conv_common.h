@@ -71,4 +71,86 @@ __device__ __forceinline__ void conv_accumulate(const float* smem, int LW, int l
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// MFMA convolution by v_mfma_f32_16x16x4f32.  A wave computes a 16 x 16
+// output block (16 columns x, kMfR rows r) as
+//   out[r][x] = sum_kx sum_j A_kx[x][j] B_kx[j][r],
+//   A_kx[x][j] = in[j][x + kx],  B_kx[j][r] = Kf[j - r][kx] (zero outside the kernel),
+// j over the staged input rows in k-steps of 4.  A comes from the staged input
+// tile, B from a per-kernel tap table tab[kx][dd] = Kf[dd - 15][kx], both in
+// LDS.  Lane layout: A[l % 16][l / 16], B[l / 16][l % 16], and
+// D[4 (l / 16) + i][l % 16] in acc[i] (x = 4 (l / 16) + i, r = l % 16).
+// A kernel column needs kMfR + kh - 1 input rows, ceil of that / 4 k-steps
+// (a compile-time count dispatched on the kernel height); the staged rows and
+// the tap table are padded with zeros to whole k-steps.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kMfR = 16;       // output rows of an MFMA block
+constexpr int kMfmaWide = 67;  // tallest kernel mfma_block dispatches (21 k-steps)
+
+__host__ __device__ inline int mf_nks(int kh) { return (kMfR + kh - 1 + 3) / 4; }  // k-steps per kx
+__host__ __device__ inline int mf_rows(int kh) { return 4 * mf_nks(kh); }
+// row pitch of the staged input: 16 mod 32 floats, so the 4 rows x 16 columns
+// of an A operand (lanes l / 16, l % 16) fall in distinct LDS banks
+__host__ __device__ inline int mf_lw(int kw) { return 16 + 32 * ((64 + kw - 1 - 16 + 31) / 32); }
+__host__ __device__ inline int mf_dd(int kh) { return mf_rows(kh) + 15; }
+__host__ inline size_t mfma_lds_bytes(int kmax, int nk) {
+  return sizeof(float) * ((size_t)mf_rows(kmax) * mf_lw(kmax) + (size_t)nk * kmax * mf_dd(kmax));
+}
+
+// All kernel columns of one output block: column kx needs NKS k-steps; the
+// operands of the next column are loaded while the MFMAs of the current one
+// issue (two register sets of NKS A and B values).
+template <int NKS>
+__device__ __forceinline__ v4f mfma_columns(const float* ta, const float* tb, int kw, int LW, int DD) {
+  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  float a0[NKS], b0[NKS], a1[NKS], b1[NKS];
+  auto load = [&](int kx, float (&a)[NKS], float (&bb)[NKS]) {
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) {
+      a[t] = ta[kx + 4 * t * LW];
+      bb[t] = tb[kx * DD + 4 * t];
+    }
+  };
+  load(0, a0, b0);
+  int kx = 0;
+  for (; kx + 1 < kw; kx += 2) {
+    load(kx + 1, a1, b1);
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
+    if (kx + 2 < kw) load(kx + 2, a0, b0);
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[t], b1[t], acc, 0, 0, 0);
+  }
+  if (kx < kw) {
+#pragma unroll
+    for (int t = 0; t < NKS; ++t) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], b0[t], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// mfma_columns with the k-step count of the kernel height (5 .. 21, kernels of
+// 1 .. kMfmaWide rows)
+__device__ __forceinline__ v4f mfma_block(int nks, const float* ta, const float* tb, int kw, int LW, int DD) {
+  switch (nks) {
+    case 5: return mfma_columns<5>(ta, tb, kw, LW, DD);
+    case 6: return mfma_columns<6>(ta, tb, kw, LW, DD);
+    case 7: return mfma_columns<7>(ta, tb, kw, LW, DD);
+    case 8: return mfma_columns<8>(ta, tb, kw, LW, DD);
+    case 9: return mfma_columns<9>(ta, tb, kw, LW, DD);
+    case 10: return mfma_columns<10>(ta, tb, kw, LW, DD);
+    case 11: return mfma_columns<11>(ta, tb, kw, LW, DD);
+    case 12: return mfma_columns<12>(ta, tb, kw, LW, DD);
+    case 13: return mfma_columns<13>(ta, tb, kw, LW, DD);
+    case 14: return mfma_columns<14>(ta, tb, kw, LW, DD);
+    case 15: return mfma_columns<15>(ta, tb, kw, LW, DD);
+    case 16: return mfma_columns<16>(ta, tb, kw, LW, DD);
+    case 17: return mfma_columns<17>(ta, tb, kw, LW, DD);
+    case 18: return mfma_columns<18>(ta, tb, kw, LW, DD);
+    case 19: return mfma_columns<19>(ta, tb, kw, LW, DD);
+    case 20: return mfma_columns<20>(ta, tb, kw, LW, DD);
+    default: return mfma_columns<21>(ta, tb, kw, LW, DD);
+  }
+}
+
 }  // namespace apx

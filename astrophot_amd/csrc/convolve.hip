@@ -31,6 +31,7 @@ constexpr int kTX = 64;   // output tile width = one wavefront
 constexpr int kRY = 4;    // outputs per thread (vertical)
 constexpr int kTY = 16;   // output tile height (4 waves x kRY)
 constexpr int kKR = 32;   // kernel rows staged per pass
+constexpr size_t kMaxLds = 160 * 1024;  // LDS a gfx950 workgroup may use
 
 __device__ __forceinline__ double block_sum256(double v, double* red) {
 #pragma unroll
@@ -291,6 +292,72 @@ __global__ __launch_bounds__(256) void convolve_fast_kernel(apx_conv_job job,
   }
 }
 
+// MFMA convolution of one job (kernels of kMfmaMin .. kMfmaWide rows): a
+// 64 x kMfR output tile per workgroup, each wave one 16 x 16 block
+// (mfma_block, conv_common.h).  The tap table of every kernel of the job is
+// built in LDS once per workgroup; per render plane the input tile is staged
+// and each of its kernels accumulated; outputs are added into the image or the
+// Jacobian column as convolve_fast_kernel does.
+constexpr int kMfmaMin = 21;  // shorter kernels waste over half of each MFMA on the band's zeros
+
+__host__ inline size_t conv_mfma_lds_bytes(int kh, int kw, int nk) {
+  return sizeof(float) * ((size_t)mf_rows(kh) * mf_lw(kw) + (size_t)nk * kw * mf_dd(kh));
+}
+
+template <int NK0>
+__global__ __launch_bounds__(256) void convolve_mfma_kernel(apx_conv_job job, const float* __restrict__ planes,
+                                                            const float* __restrict__ kernels) {
+  extern __shared__ float smem[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lm = lane & 15, lk = lane >> 4;
+  const int ox0 = blockIdx.x * 64, oy0 = blockIdx.y * kMfR;
+  const int kh = job.kh, kw = job.kw, ch = (kh - 1) / 2, cw = (kw - 1) / 2;
+  const int nks = mf_nks(kh), rows = mf_rows(kh), LW = mf_lw(kw), DD = mf_dd(kh);
+  const int cols = 64 + kw - 1;
+  const int gx0 = job.x0 + ox0 - cw, gy0 = job.y0 + oy0 - ch;
+  const int64_t npl = (int64_t)job.nx * job.ny;
+  float* tin = smem;             // rows x LW
+  float* tab = smem + rows * LW; // [k][kx][dd] = Kf_k[dd - 15][kx]
+  for (int e = threadIdx.x; e < NK0 * kw * DD; e += 256) {
+    const int dd = e % DD, kx = (e / DD) % kw, k = e / (DD * kw);
+    const int a = dd - 15;
+    tab[e] = (a >= 0 && a < kh) ? kernels[((int64_t)k * kh + a) * kw + kx] : 0.0f;
+  }
+  // this lane's 4 outputs: x = 16 w + 4 lk + i, row lm
+  const int oy = oy0 + lm;
+  const int c_first = (job.img || NK0 > 1) ? 0 : 1;
+  for (int c = c_first; c < job.nchan; ++c) {
+    const float* plane = planes + c * npl;
+    __syncthreads();
+    for (int e = threadIdx.x; e < rows * cols; e += 256) {
+      const int t = e / cols, u = e - t * cols;
+      const int gy = gy0 + t, gx = gx0 + u;
+      tin[t * LW + u] = (gy >= 0 && gy < job.ny && gx >= 0 && gx < job.nx) ? plane[(int64_t)gy * job.nx + gx]
+                                                                          : 0.0f;
+    }
+    __syncthreads();
+    const int nk = (c == 0) ? NK0 : 1;
+    for (int k = 0; k < nk; ++k) {
+      const float* ta = tin + lk * LW + 16 * w + lm;
+      const float* tb = tab + k * kw * DD + lk - lm + 15;
+      const v4f acc = mfma_block(nks, ta, tb, kw, LW, DD);
+      if (oy >= job.oh) continue;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ox = ox0 + 16 * w + 4 * lk + i;
+        if (ox >= job.ow) continue;
+        if (c == 0 && k == 0) {
+          if (job.img) job.img[(int64_t)(job.img_y0 + oy) * job.img_row + (job.img_x0 + ox)] += acc[i];
+        } else if (job.jac) {
+          float* d = job.jac + (int64_t)(job.jac_y0 + oy) * job.jac_row + (int64_t)(job.jac_x0 + ox) * job.jac_np;
+          const int col = (c == 0) ? job.dk_col[k - 1] : job.chan_col[c];
+          if (col >= 0) d[col] += acc[i];
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 int psf_shift(const float* psf, int32_t ph, int32_t pw, double sx, double sy, const double* pinv,
@@ -406,6 +473,21 @@ int convolve(const apx_conv_job* jp, const float* planes, const float* kernels, 
         return set_error(APX_E_INVALID, "convolve: centre column out of range");
   }
   const int nk0 = dk ? 3 : 1;
+  if (job.kh >= kMfmaMin && job.kh <= kMfmaWide && job.kw >= kMfmaMin && job.kw <= kMfmaWide &&
+      conv_mfma_lds_bytes(job.kh, job.kw, nk0) <= kMaxLds) {
+    const size_t lds = conv_mfma_lds_bytes(job.kh, job.kw, nk0);
+    const dim3 grid((unsigned)((job.ow + 63) / 64), (unsigned)((job.oh + kMfR - 1) / kMfR));
+    if (dk) {
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&convolve_mfma_kernel<3>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(convolve_mfma_kernel<3>, grid, dim3(256), lds, stream, job, planes, kernels);
+    } else {
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&convolve_mfma_kernel<1>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(convolve_mfma_kernel<1>, grid, dim3(256), lds, stream, job, planes, kernels);
+    }
+    return check_launch("convolve (MFMA)");
+  }
   if (job.kh <= kFastMax && job.kw <= kFastMax) {
     if (!workspace || ws_bytes < convolve_workspace_bytes(job.kh, job.kw))
       return set_error(APX_E_INVALID, "convolve: workspace too small");

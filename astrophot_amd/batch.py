@@ -284,7 +284,7 @@ class BatchFit:
         b = self.b
         log = {"conv": {}, "quad": {}, "chunk": {}}
         self.jacobian(rep, out=out, parts=1)
-        impls = [1] + ([2] if b.max_kernel <= 35 else []) + ([3] if self.sep_all and b.max_kernel <= 61 else [])
+        impls = [1] + ([2] if b.max_kernel <= 67 else []) + ([3] if self.sep_all and b.max_kernel <= 61 else [])
         best = None
         conv_grid = sorted({min(c, int(b.ntile)) for c in (1024, 2048, 4096, 8192, 16384)})  # launches clamp to ntile
         for impl in impls:

@@ -40,7 +40,7 @@ constexpr int kNChan = 6;  // value + q, PA, n, Re, Ie partials
 // skip the Ie plane and write its column from the value (times d Ie / d rep).
 constexpr int kConvChan = kNChan - 1;
 constexpr int kTR = 16;      // output rows of a batch tile (64 x kTR pixels)
-constexpr int kMfmaMax = 35; // widest kernel the MFMA convolution takes
+constexpr int kMfmaMax = kMfmaWide;  // widest kernel the MFMA convolution takes (67, up to 128 KiB LDS)
 
 enum : int32_t { PH_INIT = 0, PH_NEEDJ = 1, PH_SUB = 2, PH_DONE = 3 };
 enum : int32_t { ST_JAC = 0, ST_PROBE = 1, ST_TRIAL = 2, ST_ALL = 3, ST_ALL_JAC = 4 };
@@ -1767,9 +1767,16 @@ void conv_part(const apx_batch& b, const Scratch& s, int mode, float* out, hipSt
   const int tc = (int)std::min<int64_t>(b.ntile, b.conv_blocks > 0 ? b.conv_blocks : 2 * kPersist);
 #define APX_CONV(M)                                                                                \
   do {                                                                                             \
-    if (sep) hipLaunchKernelGGL(sconv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);     \
-    else if (mf) hipLaunchKernelGGL(conv_mfma_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out); \
-    else hipLaunchKernelGGL(conv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);          \
+    if (sep) {                                                                                     \
+      hipLaunchKernelGGL(sconv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);            \
+    } else if (mf) {                                                                               \
+      if (lds > 65536) /* wide kernels: more than the default 64 KiB of dynamic LDS */             \
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&conv_mfma_kernel<M>),                   \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
+      hipLaunchKernelGGL(conv_mfma_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);        \
+    } else {                                                                                       \
+      hipLaunchKernelGGL(conv_kernel<M>, dim3(tc), dim3(256), lds, stream, b, s, out);             \
+    }                                                                                              \
   } while (0)
   switch (mode) {
     case EPI_OUT:
@@ -1833,7 +1840,10 @@ int batch_plan(apx_batch_galaxy* gal, int32_t ngal, apx_batch* b) {
     tile += (int64_t)((G.nx + 63) / 64) * ((G.ny + kTR - 1) / kTR);
     kmax = std::max(kmax, std::max(G.ph, G.pw) + 2);
   }
-  if ((kmax <= kMfmaMax ? mfma_lds_bytes(kmax, 3) : conv_lds_bytes(kmax)) + kConvStaticLds > 65536)
+  // the VALU convolution is every batch's fallback and must fit 64 KiB; the MFMA
+  // one (kmax <= kMfmaMax) takes up to 128 KiB of the 160 KiB a workgroup has
+  if (conv_lds_bytes(kmax) + kConvStaticLds > 65536 ||
+      (kmax <= kMfmaMax && mfma_lds_bytes(kmax, 3) + kConvStaticLds > 160 * 1024))
     return set_error(APX_E_LIMIT, "batch_plan: psf too large for the batched Jacobian convolution");
   if (work > (int64_t)INT32_MAX || chunk > (int64_t)INT32_MAX || tile > (int64_t)INT32_MAX)
     return set_error(APX_E_LIMIT, "batch_plan: batch too large");
@@ -1860,7 +1870,7 @@ static int check_batch(const apx_batch* b) {
     return set_error(APX_E_LIMIT, "batch: integration settings out of range");
   if (b->max_kernel > kFastMax) return set_error(APX_E_LIMIT, "batch: psf too large");
   if (b->conv_impl < 0 || b->conv_impl > 3 || (b->conv_impl == 2 && b->max_kernel > kMfmaMax))
-    return set_error(APX_E_INVALID, "batch: conv_impl must be 0, 1, 2 or 3 (2 needs psf <= 33 px)");
+    return set_error(APX_E_INVALID, "batch: conv_impl must be 0, 1, 2 or 3 (2 needs psf <= 65 px)");
   if (b->conv_impl == 3 &&
       (!b->sep_all || !b->psf_sep || b->max_kernel > kSepMax ||
        sep_lds_bytes(b->max_kernel, true) + kConvStaticLds > 65536))

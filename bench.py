@@ -239,7 +239,7 @@ def conv_roofline(bf, truth_rep, label):
                 "value_conv": {"kernel": "sconv_kernel<0>", "algorithmic_per_launch": vbytes,
                                "avg_launch_us": val_us, "achieved_gbs": vbytes / (val_us * 1e-6) / 1e9}}
     achieved = jflops / (jac_us * 1e-6)
-    kernel = "conv_mfma_kernel<1>" if (bf.b.conv_impl != 1 and bf.b.max_kernel <= 35) else "conv_kernel<1>"
+    kernel = "conv_mfma_kernel<1>" if (bf.b.conv_impl != 1 and bf.b.max_kernel <= 67) else "conv_kernel<1>"
     t = measured_traffic(kernel, len(bf.gal_host))
     return {"bound": "valu", "kernel": f"{kernel} (batched Jacobian-stage convolution, {label})",
             "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",

@@ -303,7 +303,7 @@ typedef struct apx_batch {
   void* state;                 /* ngal x apx_batch_state_bytes() */
   void* scratch;               /* apx_batch_scratch_bytes(batch) */
   /* launch shape (0 = built-in default); set by the caller's autotuner */
-  int32_t conv_impl;           /* 1 = VALU convolution, 2 = MFMA (kernels <= 35 px),
+  int32_t conv_impl;           /* 1 = VALU convolution, 2 = MFMA (kernels <= 67 px),
                                   3 = separable (sep_all, kernels <= 61 px) */
   int32_t conv_blocks;         /* workgroups of the convolution launches */
   int32_t quad_blocks;         /* workgroups of the level-2/3 quadrature launches */

@@ -274,3 +274,24 @@ def test_quadrature_rerun_and_work_counters():
     out = torch.zeros_like(ref)
     bf.render(rep, out=out, parts=2)
     assert torch.equal(out, ref)
+
+
+def test_mfma_convolution_wide_psf():
+    """A 65 x 65 Moffat PSF (67-px shifted kernel, 21 k-steps a column, 128 KiB
+    of LDS a workgroup): the MFMA Jacobian-stage convolution agrees with the
+    VALU one to float32 summation order -- 4489-term sums, so the bar is the
+    image parity bar, 1e-5 of each column's max (about sqrt(4489) float32
+    epsilons), rather than the 1e-6 of the short kernels above."""
+    truths, shapes, rng = make_cutouts(3, seed=23, sizes=(70, 90))
+    psf = ap.utils.moffat_psf(3.5, 2.9, 65, 1.0)
+    cuts = [Cutout(np.zeros((s, s)), np.ones((s, s)), psf) for s, _ in shapes]
+    bf = BatchFit(cuts)
+    assert not bf.sep_all and bf.b.max_kernel == 67
+    rep = torch.tensor(values_to_rep(np.stack(truths)), dtype=torch.float64, device="cuda")
+    bf.b.conv_impl = 1
+    ref_m, ref_j = [t.clone() for t in bf.jacobian(rep)]
+    bf.b.conv_impl = 2
+    m, j = bf.jacobian(rep)
+    assert float((m - ref_m).abs().max() / ref_m.abs().max()) < 1e-5
+    scale = ref_j.abs().amax(dim=0)
+    assert float(((j - ref_j).abs().amax(dim=0) / scale).max()) < 1e-5

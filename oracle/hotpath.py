@@ -51,8 +51,13 @@ chi2                       reference KATs ``test_utils.py:55-121`` (4.5, 0.75,
                            5, 5/7, 9, 1.5, 10, 10/7)
 LM damping / solve         numpy.linalg.solve of the damped matrix; fit recovers
                            truth on noiseless data
-LM accept/reject sequence  parity unpinned beyond the above: restated line by
-                           line from ``fit/lm.py:248-493``
+LM accept/reject sequence  closed-form trajectories on linear models
+                           (tests/test_oracle_lm.py): accept path (states,
+                           chi^2 and L per step while each step's first trial
+                           gains over 10 %), reject path (L x 11 per refused
+                           trial to the 1e9 cap, trial points, "fail"); the
+                           mixed accept-after-reject sequences of non-linear
+                           fits stay unpinned
 =========================  ==================================================
 """
 

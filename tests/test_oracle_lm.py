@@ -1,0 +1,137 @@
+"""Pins of the oracle's LM control flow (fit/lm.py:248-357, 427-493) on
+problems whose trajectory follows in closed form.  CPU only.
+
+The model is linear, m(theta) = A theta, so chi^2 is a quadratic with
+minimum theta_hat (least squares) and residual floor S.  With e = theta_hat -
+theta the gradient is H e (H = A^T W A), the second-order correction rpp is
+zero, and a step taken at damping L is h = D_L^-1 H e, where D_L is the damped
+matrix of fit/lm.py:361-371 (off-diagonal of H divided by 1 + L, diagonal
+H_ii + L (1 + H_ii)).  Hence:
+
+* accept path: while each step's first trial lowers chi^2 by more than 10 %,
+  a step makes exactly one trial, LM keeps that trial's L, and fit() then
+  divides it by Ldn = 9.  The k-th state is theta_hat - e_k with
+  e_{k+1} = (I - D_{L_k}^-1 H) e_k, L_k = 9^-k, and chi^2_k =
+  (e_k^T H e_k + S) / ndf;
+* reject path: with the Jacobian's sign flipped every trial moves away from
+  the minimum.  A trial is then refused either by the curvature test
+  (rho = |a| / |h| = (2/d) H / (H + L (1 + H)) > 1 for one parameter) or as
+  worse; both raise L by Lup = 11 until it reaches the 1e9 cap, which ends
+  the step with no improvement and the fit with "fail".
+
+A dropped term, a wrong constant (Lup, Ldn, L0, d, the 10 % rule), the L kept
+before rather than after the trial, or a wrong sign changes these figures.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import hotpath as H
+
+F64 = torch.float64
+
+
+class LinearModel:
+    """m(theta) = A theta with the oracle Model's interface for LM."""
+
+    def __init__(self, A, flip_jacobian=False):
+        self.A = torch.as_tensor(A, dtype=F64)
+        self.nparams = self.A.shape[1]
+        self.flip = flip_jacobian
+        self.calls = []
+
+    def vector_representation(self):
+        return torch.zeros(self.nparams, dtype=F64)
+
+    def forward(self, rep):
+        self.calls.append(rep.clone())
+        return self.A @ rep
+
+    def jacobian(self, rep):
+        return -self.A if self.flip else self.A.clone()
+
+
+def damped(Hm, L):
+    """fit/lm.py:361-371: the damped normal matrix."""
+    off = Hm / (1 + L)
+    np.fill_diagonal(off, np.diag(Hm) + L * (1 + np.diag(Hm)))
+    return off
+
+
+def problem(ncol, seed):
+    rng = np.random.default_rng(seed)
+    N = 400
+    x = np.linspace(0.0, 1.0, N)
+    A = np.stack([np.ones(N), x, x ** 2][:ncol], axis=1)
+    theta_true = np.array([3.0, -2.0, 1.5])[:ncol]
+    Y = A @ theta_true + rng.normal(size=N) * 1e-3
+    theta_hat, *_ = np.linalg.lstsq(A, Y, rcond=None)
+    S = float(np.sum((Y - A @ theta_hat) ** 2))
+    return A, Y, theta_hat, S, N - ncol
+
+
+@pytest.mark.parametrize("ncol,seed", [(1, 0), (2, 1), (3, 2)])
+def test_lm_accept_path_closed_form(ncol, seed):
+    A, Y, theta_hat, S, ndf = problem(ncol, seed)
+    Hm = A.T @ A
+    start = theta_hat + np.array([10.0, -7.0, 4.0])[:ncol]
+    lm = H.LM(LinearModel(A), torch.tensor(Y), initial_state=torch.tensor(start)).fit()
+    # closed-form prefix: steps whose single trial drops chi^2 by over 10 %
+    e = theta_hat - start
+    chi = [(e @ Hm @ e + S) / ndf]
+    Ls = [1.0]
+    states = [start]
+    L = 1.0
+    while True:
+        e_next = e - np.linalg.solve(damped(Hm, L), Hm @ e)
+        c = (e_next @ Hm @ e_next + S) / ndf
+        if (c - chi[-1]) / chi[-1] >= -0.11:
+            break
+        e = e_next
+        chi.append(c)
+        Ls.append(L)          # the L of the accepted trial is kept ...
+        states.append(theta_hat - e)
+        L = L / 9.0           # ... and fit() lowers it once
+    K = len(chi)
+    assert K >= 3
+    assert np.allclose(lm.loss_history[:K], chi, rtol=1e-10, atol=0)
+    assert np.allclose(lm.L_history[:K], Ls, rtol=1e-12, atol=0)
+    assert np.allclose(np.array(lm.lambda_history[:K]), np.array(states), rtol=1e-10, atol=1e-12)
+    # and it ends at the least-squares solution: either by the convergence
+    # test, or (fit/lm.py:355, 455-458) with "no step improves chi^2"
+    # once the history already sits on the floor S / ndf
+    if not lm.message.startswith("success"):
+        assert lm.message == "fail. Could not find step to improve Chi^2"
+        assert lm.loss_history[-1] == pytest.approx(S / ndf, rel=1e-9)
+    assert np.allclose(lm.res().numpy(), theta_hat, rtol=0, atol=1e-6)
+    assert min(lm.loss_history) == pytest.approx(S / ndf, rel=1e-6)
+
+
+def test_lm_reject_path_raises_damping_to_the_cap():
+    A, Y, theta_hat, S, ndf = problem(1, 3)
+    start = theta_hat + 10.0
+    Hs = float(A[:, 0] @ A[:, 0])
+    model = LinearModel(A, flip_jacobian=True)
+    lm = H.LM(model, torch.tensor(Y), initial_state=torch.tensor(start))
+    lm.fit()
+    # L runs 1, 11, 121, ... ; the curvature test refuses the first trials,
+    # the chi^2 comparison the rest, and the Lup that reaches 1e9 ends the step
+    rho = [20.0 * Hs / (Hs + 11.0 ** k * (1 + Hs)) for k in range(9)]
+    assert rho[0] > 1 and rho[-1] < 1
+    trials = 1 + math.ceil(math.log(1e9) / math.log(11.0)) - 1
+    assert trials == 9
+    assert lm.n_forward == 1 + 1 + 2 * trials
+    # each trial probes start + d h, then evaluates start + h, with the
+    # flipped gradient -H e and L = 11^k (the reversed step, damped)
+    e = theta_hat[0] - start[0]
+    for k in range(trials):
+        h = -Hs * e / (Hs + 11.0 ** k * (1 + Hs))
+        assert float(model.calls[2 + 2 * k][0]) - start[0] == pytest.approx(0.1 * h, rel=1e-9)
+        assert float(model.calls[3 + 2 * k][0]) - start[0] == pytest.approx(h, rel=1e-9)
+    assert lm.L == 1e9
+    assert lm.message == "fail. Could not find step to improve Chi^2"
+    assert len(lm.loss_history) == 1 and lm.loss_history[0] == pytest.approx(
+        (Hs * 100.0 + S) / ndf, rel=1e-12)

@@ -444,13 +444,22 @@ __device__ __forceinline__ int band_test(float res, float centre, double thr) {
   return -1;
 }
 
+// The chunk list and the per-fit records arrive again as read-only,
+// non-aliasing arguments: the compiler then reads them with scalar loads, whose
+// wait counter is separate from the vector memory counter.  Read through the
+// structs' pointers they are vector loads, and the wait for them at the top of
+// each chunk also waits for the previous chunk's plane stores to complete.
+
 // midpoint samples of every working pixel (and their parameter partials)
 template <bool GRAD>
-__global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s) {
+__global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s,
+                                                      const int2* __restrict__ chunks,
+                                                      const apx_batch_galaxy* __restrict__ gal,
+                                                      const GalEval* __restrict__ ev) {
   for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
-    const int2 e = s.chunk_list[k];
-    const apx_batch_galaxy G = b.gal[e.x];
-    const GalEval E = s.ev[e.x];
+    const int2 e = chunks[k];
+    const apx_batch_galaxy G = gal[e.x];
+    const GalEval E = ev[e.x];
     const int pix = e.y * kChunk + threadIdx.x;
     if (pix >= E.nxw * E.nyw) continue;
     float X, Y, gr[APX_NSLOT];
@@ -546,12 +555,15 @@ __device__ __forceinline__ void wave_append(bool flag, int32_t* counter, int32_t
 
 // float32 selection of every listed chunk, one lane per pixel
 template <bool GRAD, int QN>
-__global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, QuadTab t) {
+__global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, QuadTab t,
+                                                    const int2* __restrict__ chunks,
+                                                    const apx_batch_galaxy* __restrict__ gal,
+                                                    const GalEval* __restrict__ ev) {
   const int lane = threadIdx.x & 63;
   for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
-    const int2 e = s.chunk_list[k];
-    const apx_batch_galaxy G = b.gal[e.x];
-    const GalEval E = s.ev[e.x];
+    const int2 e = chunks[k];
+    const apx_batch_galaxy G = gal[e.x];
+    const GalEval E = ev[e.x];
     const int pix = e.y * kChunk + threadIdx.x;
     const int res = pix < E.nxw * E.nyw ? select_pixel<GRAD, false, QN>(b, s, G, E, t, pix) : PX_DONE;
     wave_append(res == PX_DEEPER, &s.gal_cnt[e.x], s.deep + G.work_off, pix, lane);
@@ -1698,11 +1710,13 @@ template <int QN, int QG>
 void render_launches(const apx_batch& b, const Scratch& s, const QuadTab& t, bool jac, int pc,
                      hipStream_t stream) {
   if (jac) {
-    hipLaunchKernelGGL(init_kernel<true>, dim3(pc), dim3(kChunk), 0, stream, b, s);
-    hipLaunchKernelGGL((select_kernel<true, QN>), dim3(pc), dim3(256), 0, stream, b, s, t);
+    hipLaunchKernelGGL(init_kernel<true>, dim3(pc), dim3(kChunk), 0, stream, b, s, s.chunk_list, b.gal, s.ev);
+    hipLaunchKernelGGL((select_kernel<true, QN>), dim3(pc), dim3(256), 0, stream, b, s, t, s.chunk_list, b.gal,
+                       s.ev);
   } else {
-    hipLaunchKernelGGL(init_kernel<false>, dim3(pc), dim3(kChunk), 0, stream, b, s);
-    hipLaunchKernelGGL((select_kernel<false, QN>), dim3(pc), dim3(256), 0, stream, b, s, t);
+    hipLaunchKernelGGL(init_kernel<false>, dim3(pc), dim3(kChunk), 0, stream, b, s, s.chunk_list, b.gal, s.ev);
+    hipLaunchKernelGGL((select_kernel<false, QN>), dim3(pc), dim3(256), 0, stream, b, s, t, s.chunk_list, b.gal,
+                       s.ev);
   }
   // float64 redo of undecided selections, then the level-2/3 quadrature of
   // the deep list, then its undecided pixels in float64

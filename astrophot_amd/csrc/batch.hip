@@ -456,19 +456,29 @@ __global__ __launch_bounds__(kChunk) void init_kernel(apx_batch b, Scratch s,
                                                       const int2* __restrict__ chunks,
                                                       const apx_batch_galaxy* __restrict__ gal,
                                                       const GalEval* __restrict__ ev) {
-  for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
+  // a contiguous run of chunks per workgroup: a fit's chunks are listed
+  // together, so its record is read again only when the fit changes
+  const int n = s.cnt[1], per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+  int cur = -1;
+  int64_t work_off = 0;
+  GalEval E;
+  for (int k = k0; k < k1; ++k) {
     const int2 e = chunks[k];
-    const apx_batch_galaxy G = gal[e.x];
-    const GalEval E = ev[e.x];
+    if (e.x != cur) {
+      cur = e.x;
+      work_off = gal[e.x].work_off;
+      E = ev[e.x];
+    }
     const int pix = e.y * kChunk + threadIdx.x;
     if (pix >= E.nxw * E.nyw) continue;
     float X, Y, gr[APX_NSLOT];
     pixel_xy(E.pf, E.ci, E.cj, pix % E.nxw, pix / E.nxw, X, Y);
     const float val = sersic_f<GRAD>(E.pf, X, Y, 1.0f, gr);
-    s.mid[G.work_off + pix] = val;
+    s.mid[work_off + pix] = val;
     if (GRAD) {
       // planes 1..5: q, PA, n, Re, Ie in representation units
-      float* pl = b.planes + G.work_off + pix;
+      float* pl = b.planes + work_off + pix;
 #pragma unroll
       for (int c = 1; c < kNChan; ++c) pl[c * b.nwork] = gr[c + 1] * (float)E.dvdr[c + 1];
     }

@@ -570,10 +570,19 @@ __global__ __launch_bounds__(256) void select_kernel(apx_batch b, Scratch s, Qua
                                                     const apx_batch_galaxy* __restrict__ gal,
                                                     const GalEval* __restrict__ ev) {
   const int lane = threadIdx.x & 63;
-  for (int k = blockIdx.x; k < s.cnt[1]; k += gridDim.x) {
+  // contiguous chunk runs, as in init_kernel
+  const int n = s.cnt[1], per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int k0 = blockIdx.x * per, k1 = min(n, k0 + per);
+  int cur = -1;
+  apx_batch_galaxy G;
+  GalEval E;
+  for (int k = k0; k < k1; ++k) {
     const int2 e = chunks[k];
-    const apx_batch_galaxy G = gal[e.x];
-    const GalEval E = ev[e.x];
+    if (e.x != cur) {
+      cur = e.x;
+      G = gal[e.x];
+      E = ev[e.x];
+    }
     const int pix = e.y * kChunk + threadIdx.x;
     const int res = pix < E.nxw * E.nyw ? select_pixel<GRAD, false, QN>(b, s, G, E, t, pix) : PX_DONE;
     wave_append(res == PX_DEEPER, &s.gal_cnt[e.x], s.deep + G.work_off, pix, lane);

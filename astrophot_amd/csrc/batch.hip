@@ -1267,6 +1267,8 @@ __global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
   __shared__ double redH[4][256];
   __shared__ double redg[4][16];
   __shared__ double redc[4];
+  __shared__ float sJ[4][NP][64];  // one tile row of J columns, per wave
+  __shared__ float sW[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r16 = lane & 15, kq = lane >> 4;
   for (int k = blockIdx.x; k < s.cnt[2]; k += gridDim.x) {
@@ -1275,33 +1277,50 @@ __global__ __launch_bounds__(256) void normal_kernel(apx_batch b, Scratch s) {
     const int ntx = (G.nx + 63) / 64;
     const int ox0 = (e.y % ntx) * 64, oy0 = (e.y / ntx) * kTR;
     v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    float gpart = 0.0f;
+    float gl[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) gl[i] = 0.0f;
     double cacc = 0.0;
-    for (int stp = 0; stp < 16 * kTR / 4; ++stp) {
-      const int pt = wv * 16 * kTR + stp * 4 + kq;  // tile pixel
-      const int px = ox0 + (pt & 63), py = oy0 + (pt >> 6);
-      float a = 0.0f, wt = 0.0f, res = 0.0f;
+    // wave wv takes tile rows 4 wv .. 4 wv + 3: each row is read coalesced
+    // (one pixel a lane), staged in LDS, and fed to the MFMA 4 pixels a step
+    for (int r = 0; r < kTR / 4; ++r) {
+      const int px = ox0 + lane, py = oy0 + wv * (kTR / 4) + r;
+      float wt = 0.0f, res = 0.0f, jv[NP];
+#pragma unroll
+      for (int i = 0; i < NP; ++i) jv[i] = 0.0f;
       if (px < G.nx && py < G.ny) {
         const int64_t p = G.pix_off + (int64_t)py * G.nx + px;
         const float m = b.model0[p], y = b.data[p];
         wt = b.weight[p];
         res = y - m;
-        if (r16 < NP) a = b.jac[(int64_t)r16 * b.npix + p];
-        if (r16 == 0) {
-          const double dd = (double)y - (double)m;
-          cacc += (double)wt * dd * dd;
-        }
+#pragma unroll
+        for (int i = 0; i < NP; ++i) jv[i] = b.jac[(int64_t)i * b.npix + p];
+        const double dd = (double)y - (double)m;
+        cacc += (double)wt * dd * dd;
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wt * a, acc, 0, 0, 0);
-      gpart = fmaf(a * wt, res, gpart);
+#pragma unroll
+      for (int i = 0; i < NP; ++i) {
+        sJ[wv][i][lane] = jv[i];
+        gl[i] = fmaf(jv[i] * wt, res, gl[i]);
+      }
+      sW[wv][lane] = wt;
+      __syncthreads();
+      // pixels 4 stp + kq of the row, in the order of the row
+      for (int stp = 0; stp < 16; ++stp) {
+        const int q = stp * 4 + kq;
+        const float a = r16 < NP ? sJ[wv][r16][q] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, sW[wv][q] * a, acc, 0, 0, 0);
+      }
+      __syncthreads();  // the row's LDS is rewritten by the next row
     }
     // D[4 kq + i][r16] = acc[i]
 #pragma unroll
     for (int i = 0; i < 4; ++i) redH[wv][(4 * kq + i) * 16 + r16] = (double)acc[i];
-    double gd = (double)gpart;
-    gd += __shfl_xor(gd, 16, 64);
-    gd += __shfl_xor(gd, 32, 64);
-    if (lane < 16) redg[wv][lane] = gd;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const double gd = wsum((double)gl[i]);
+      if (lane == 0) redg[wv][i] = gd;
+    }
     cacc = wsum(cacc);
     if (lane == 0) redc[wv] = cacc;
     __syncthreads();
